@@ -1,0 +1,173 @@
+"""ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads oracle/_build/libslosh_oracle.so (built by oracle/build.sh). Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module;
+the product path (slosh_amd) never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from slosh_amd import layouts as L
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libslosh_oracle.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["sh", os.path.join(_HERE, "build.sh")])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        l = C.CDLL(_LIB_PATH)
+        vp = C.c_void_p
+        l.so_create.restype = vp
+        l.so_create.argtypes = [vp, C.c_size_t, vp, vp, vp, vp, vp, C.c_size_t, vp, vp, C.c_float, C.c_uint32, C.c_int]
+        l.so_destroy.argtypes = [vp]
+        l.so_step.argtypes = [vp]
+        l.so_step.restype = C.c_int
+        l.so_set_threads.argtypes = [C.c_int]
+        for name in ("so_read_positions", "so_read_kinematics", "so_read_def_grad", "so_read_models",
+                     "so_read_blocks", "so_read_nodes"):
+            getattr(l, name).argtypes = [vp, vp]
+        l.so_read_bodies.argtypes = [vp, vp, vp]
+        l.so_num_blocks.argtypes = [vp]
+        l.so_num_blocks.restype = C.c_uint32
+        l.so_svd3.argtypes = [vp, C.c_int, vp, vp, vp]
+        l.so_model_update.argtypes = [vp, vp, vp, C.c_int]
+        l.so_prefix_sum.argtypes = [vp, C.c_size_t]
+        l.so_pack_key.argtypes = [C.c_int32, C.c_int32, C.c_int32]
+        l.so_pack_key.restype = C.c_uint32
+        l.so_hash.argtypes = [C.c_uint32]
+        l.so_hash.restype = C.c_uint32
+        _lib = l
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """Sequential restatement of MpmData + MpmPipeline::launch_step."""
+
+    def __init__(self, scene, svd_mode=1, threads=0):
+        l = lib()
+        if threads:
+            l.so_set_threads(threads)
+        p = scene.particles
+        self._keep = [np.ascontiguousarray(p[k]) for k in ("pos4", "kin", "def_grad", "props", "models")]
+        self._params = np.array([scene.params], dtype=L.SIM_PARAMS)
+        self._bodies = np.ascontiguousarray(scene.bodies)
+        self._mats = np.ascontiguousarray(scene.materials)
+        self.n = scene.n
+        self.nb = len(self._bodies)
+        self._s = l.so_create(_p(self._params), self.n, *[_p(a) for a in self._keep], self.nb,
+                              _p(self._bodies), _p(self._mats), float(scene.cell_width),
+                              scene.grid_capacity, svd_mode)
+        if not self._s:
+            raise MemoryError("oracle allocation failed")
+
+    def __del__(self):
+        if getattr(self, "_s", None):
+            lib().so_destroy(self._s)
+            self._s = None
+
+    def step(self, k=1):
+        for _ in range(k):
+            rc = lib().so_step(self._s)
+            if rc != 0:
+                raise RuntimeError("oracle: hash map overflow")
+
+    def positions(self):
+        out = np.zeros((self.n, 4), np.float32)
+        lib().so_read_positions(self._s, _p(out))
+        return out
+
+    def kinematics(self):
+        out = np.zeros(self.n, L.KINEMATICS)
+        lib().so_read_kinematics(self._s, _p(out))
+        return out
+
+    def def_grad(self):
+        out = np.zeros((self.n, 3, 4), np.float32)
+        lib().so_read_def_grad(self._s, _p(out))
+        return out
+
+    def models(self):
+        out = np.zeros((self.n, 13), np.uint32)
+        lib().so_read_models(self._s, _p(out))
+        return out
+
+    def bodies(self):
+        poses = np.zeros(self.nb, L.SIM3)
+        vels = np.zeros(self.nb, L.VELOCITY)
+        if self.nb:
+            lib().so_read_bodies(self._s, _p(poses), _p(vels))
+        return poses, vels
+
+    def blocks(self):
+        nb = lib().so_num_blocks(self._s)
+        out = np.zeros(nb, L.BLOCK_INFO)
+        if nb:
+            lib().so_read_blocks(self._s, _p(out))
+        return out
+
+    def nodes(self):
+        nb = lib().so_num_blocks(self._s)
+        out = np.zeros((nb * 64, 4), np.float32)
+        if nb:
+            lib().so_read_nodes(self._s, _p(out))
+        return out
+
+
+def svd3(a, f64=True):
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(9)
+    u = np.zeros(9)
+    s = np.zeros(3)
+    v = np.zeros(9)
+    lib().so_svd3(_p(a), 1 if f64 else 0, _p(u), _p(s), _p(v))
+    return u.reshape(3, 3), s, v.reshape(3, 3)
+
+
+def model_update(model, def_grad, svd_mode=1):
+    m = np.array(model, dtype=np.uint32).copy()
+    f = np.ascontiguousarray(def_grad, dtype=np.float32).reshape(9).copy()
+    st = np.zeros(9, np.float32)
+    lib().so_model_update(_p(m), _p(f), _p(st), svd_mode)
+    return m, f.reshape(3, 3), st.reshape(3, 3)
+
+
+def prefix_sum(v):
+    a = np.ascontiguousarray(v, dtype=np.uint32).copy()
+    lib().so_prefix_sum(_p(a), a.size)
+    return a
+
+
+def pack_key(x, y, z):
+    return lib().so_pack_key(x, y, z)
+
+
+def hash_key(k):
+    return lib().so_hash(k)
+
+
+def grid_update_node(bodies, mats, h, params, cell_pos, mvm):
+    l = lib()
+    l.so_grid_update_node.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_float, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]
+    b = np.ascontiguousarray(np.asarray(bodies, dtype=L.BODY).reshape(-1))
+    m = np.ascontiguousarray(np.asarray(mats, dtype=L.BOUNDARY_COND).reshape(-1))
+    pr = np.array([params], dtype=L.SIM_PARAMS)
+    cp = np.ascontiguousarray(cell_pos, dtype=np.float32)
+    io = np.ascontiguousarray(mvm, dtype=np.float32).copy()
+    nd = np.zeros(4, np.float32)
+    l.so_grid_update_node(_p(b), _p(m), len(b), float(h), _p(pr), _p(cp), _p(io), _p(nd))
+    return io, nd

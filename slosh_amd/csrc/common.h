@@ -1,0 +1,122 @@
+// Shared device definitions of the gfx950 MPM substep.
+//
+// Grid vocabulary follows the reference (shaders/slosh/grid/grid.slang):
+// blocks of 4x4x4 nodes, a virtual block id per block, a hash map from the
+// packed block key to a dense "header id" (hid) assigned per substep.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SLOSH_NONE 0xffffffffu
+
+// Number of nodes per block (grid.slang:31) and of nodes in a block's
+// quadratic-stencil footprint (the block plus the +1 halo: 6 x 6 x 6).
+#define NODES_PER_BLOCK 64
+#define TILE_NODES 216
+
+// Counter slots in the per-data device counter array.
+#define CTR_NUM_BLOCKS 0
+#define CTR_SORTED_ALLOC 1
+#define CTR_ERRORS 2
+#define ERR_HASH_FULL 1u
+#define ERR_LOOKUP 2u
+
+// Neighbour table per block (16 words): [0..6] the +1 neighbours in the
+// reference's associated-block order (grid.slang:371-379, offsets 1..7),
+// [7] the block's header id in the previous substep's hash map,
+// [8..14] the -1 neighbours (same offsets, negated), [15] unused.
+#define NBR_STRIDE 16
+#define NBR_PREV 7
+#define NBR_MINUS 8
+
+struct StepParams {
+    float gravity[3];
+    float dt;
+    float h;      // cell width
+    float inv_h;  // 1.0f / h, as p2g_scatter_style.slang:54
+    uint32_t cap; // hash capacity (power of two) == max blocks
+    uint32_t n;   // particle count
+    uint32_t batched;     // 0: pack_key of grid.slang:83-90; 1: scene << 18 | 6-bit x, y, z (offset 16)
+    uint32_t num_bodies;
+    uint32_t has_force;   // force_dt may be non-zero
+    uint32_t has_sand;    // DP state arrays allocated
+    uint32_t coll_cache;  // use the per-node collision cache (moving colliders)
+    uint32_t cur;         // ping-pong index of the current hash table / cache
+    uint32_t svd_f64;     // SVD-based stress / plasticity in FP64 (1) or FP32 (0)
+};
+
+// ---------------------------------------------------------------------------
+// Block keys.
+
+// pack_key, grid.slang:83-90
+__device__ __host__ inline uint32_t pack_key_single(int x, int y, int z) {
+    return ((uint32_t)(x + 0x3ff) & 0x7ffu) | (((uint32_t)(y + 0x1ff) & 0x3ffu) << 11) |
+           (((uint32_t)(z + 0x3ff) & 0x7ffu) << 21);
+}
+
+__device__ __host__ inline uint32_t pack_key_batched(uint32_t scene, int x, int y, int z) {
+    return (scene << 18) | (((uint32_t)(z + 16) & 63u) << 12) | (((uint32_t)(y + 16) & 63u) << 6) |
+           ((uint32_t)(x + 16) & 63u);
+}
+
+__device__ __host__ inline uint32_t pack_key(uint32_t batched, uint32_t scene, int x, int y, int z) {
+    return batched ? pack_key_batched(scene, x, y, z) : pack_key_single(x, y, z);
+}
+
+__device__ __host__ inline void unpack_key(uint32_t batched, uint32_t k, int *x, int *y, int *z, uint32_t *scene) {
+    if (batched) {
+        *x = (int)(k & 63u) - 16;
+        *y = (int)((k >> 6) & 63u) - 16;
+        *z = (int)((k >> 12) & 63u) - 16;
+        *scene = k >> 18;
+    } else {
+        *x = (int)(k & 0x7ffu) - 0x3ff;
+        *y = (int)((k >> 11) & 0x3ffu) - 0x1ff;
+        *z = (int)((k >> 21) & 0x7ffu) - 0x3ff;
+        *scene = 0;
+    }
+}
+
+// Murmur3-style mixing, grid.slang:93-100
+__device__ __host__ inline uint32_t hash_key(uint32_t k) {
+    k *= 0xcc9e2d51u;
+    k = (k << 15) | (k >> 17);
+    k *= 0x1b873593u;
+    return k;
+}
+
+// Associated cell (round(x / h) - 1, half-to-even as WGSL round) and the block
+// it belongs to: block_associated_to_point (grid.slang:346-355) and
+// associated_cell_index_in_block_off_by_one (particle.slang:79-87).
+__device__ inline void assoc_block_cell(float h, float px, float py, float pz, int *bx, int *by, int *bz,
+                                        uint32_t *cell) {
+    float ax = rintf(px / h) - 1.0f;
+    float ay = rintf(py / h) - 1.0f;
+    float az = rintf(pz / h) - 1.0f;
+    float fx = floorf(ax / 4.0f), fy = floorf(ay / 4.0f), fz = floorf(az / 4.0f);
+    *bx = (int)fx;
+    *by = (int)fy;
+    *bz = (int)fz;
+    uint32_t lx = (uint32_t)(ax - fx * 4.0f);
+    uint32_t ly = (uint32_t)(ay - fy * 4.0f);
+    uint32_t lz = (uint32_t)(az - fz * 4.0f);
+    *cell = lx + 4u * ly + 16u * lz;
+}
+
+// Bounded open-addressing lookup (find_block_header_id, grid.slang:176-198).
+__device__ inline uint32_t hash_find(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+                                     uint32_t cap, uint32_t key) {
+    uint32_t slot = hash_key(key) & (cap - 1u);
+    for (uint32_t k = 0; k < cap; k++) {
+        uint32_t st = keys[slot];
+        if (st == key) return vals[slot];
+        if (st == SLOSH_NONE) return SLOSH_NONE;
+        slot = (slot + 1u) & (cap - 1u);
+    }
+    return SLOSH_NONE;
+}
+
+// Wave-level helpers (64-lane wavefronts).
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+__device__ inline uint64_t lanemask_lt() { return __lanemask_lt(); }

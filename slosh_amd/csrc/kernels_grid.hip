@@ -1,0 +1,269 @@
+// Sparse-grid build for one substep on gfx950: hashed block insertion,
+// active-block compaction and the particle sort by (block, cell) key.
+//
+// Replaces WgGrid::launch_sort (src/grid/grid.rs:78-238) and the sort kernels of
+// shaders/slosh/grid/{grid,sort}.slang. The active-block set is the reference's
+// (primary blocks of every particle plus their 7 "+1" neighbours,
+// sort.slang:50-92); header ids are dense but their order differs (the
+// reference's order depends on atomic order as well).
+//
+// Kernel sequence (one launch each, all grid-stride / upper-bound grids so the
+// sequence is graph-capturable without reading counts back):
+//   k_reset_table      hash slots <- NONE, per-step counters <- 0
+//   k_touch_primary    each particle inserts its primary block; one CAS per
+//                      distinct key per wavefront (wave-level dedup)
+//   k_touch_neighbors  each primary slot inserts its 7 +1 neighbours
+//   k_compact          occupied slots -> dense header ids (ballot + popcount,
+//                      one atomic per wavefront); zeroes the block's cell counters
+//   k_count            per-particle hash lookup (dedup per wave) and per-(block,cell)
+//                      rank via one aggregated atomic per distinct key per wave
+//   k_block_prepare    per block: wave-level exclusive scan of its 64 cell counts,
+//                      block range allocation, +1/-1 neighbour and previous-step ids
+//   k_scatter          sorted slot = cell start + rank
+#include "common.h"
+
+namespace {
+
+constexpr int ASSOC[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1},
+                             {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+
+__device__ inline void hash_insert(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals, uint32_t cap,
+                                   uint32_t key, bool primary, uint32_t *__restrict__ counters) {
+    // insertion_index (grid.slang:118-174). HIP's atomicCAS is a strong CAS.
+    uint32_t slot = hash_key(key) & (cap - 1u);
+    for (uint32_t k = 0; k < cap; k++) {
+        uint32_t old = atomicCAS(&keys[slot], SLOSH_NONE, key);
+        if (old == SLOSH_NONE) {
+            if (primary) vals[slot] = 1u;
+            return;
+        }
+        if (old == key) return;
+        slot = (slot + 1u) & (cap - 1u);
+    }
+    atomicOr(&counters[CTR_ERRORS], ERR_HASH_FULL);
+}
+
+__device__ inline uint32_t particle_key(const float4 &p, float h, uint32_t batched, uint32_t *cell) {
+    int bx, by, bz;
+    assoc_block_cell(h, p.x, p.y, p.z, &bx, &by, &bz, cell);
+    uint32_t scene = __float_as_uint(p.w) >> 1;
+    return pack_key(batched, scene, bx, by, bz);
+}
+
+__global__ __launch_bounds__(256) void k_reset_table(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                     uint32_t cap, uint32_t *__restrict__ counters) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += stride) {
+        keys[s] = SLOSH_NONE;
+        vals[s] = 0u;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[CTR_NUM_BLOCKS] = 0u;
+        counters[CTR_SORTED_ALLOC] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_touch_primary(const float4 *__restrict__ pos, uint32_t n, float h,
+                                                       uint32_t batched, uint32_t *__restrict__ keys,
+                                                       uint32_t *__restrict__ vals, uint32_t cap,
+                                                       uint32_t *__restrict__ counters) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = i < n;
+    uint32_t key = 0u, cell;
+    if (active) key = particle_key(pos[i], h, batched, &cell);
+    // One insertion per distinct key of the wavefront.
+    while (true) {
+        uint64_t m = __ballot(active);
+        if (m == 0) break;
+        int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t lk = __shfl(key, leader);
+        if (active && key == lk) {
+            if ((int)lane_id() == leader) hash_insert(keys, vals, cap, lk, true, counters);
+            active = false;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_touch_neighbors(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                         uint32_t cap, uint32_t batched,
+                                                         uint32_t *__restrict__ counters) {
+    // touch_neighbor_blocks (sort.slang:75-92): only slots flagged primary by
+    // k_touch_primary expand; slots this kernel fills keep flag 0.
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += stride) {
+        uint32_t k = keys[s];
+        if (k == SLOSH_NONE || vals[s] != 1u) continue;
+        int x, y, z;
+        uint32_t scene;
+        unpack_key(batched, k, &x, &y, &z, &scene);
+        for (int o = 1; o < 8; o++)
+            hash_insert(keys, vals, cap, pack_key(batched, scene, x + ASSOC[o][0], y + ASSOC[o][1], z + ASSOC[o][2]),
+                        false, counters);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact(const uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                 uint32_t cap, uint32_t *__restrict__ hdr_key,
+                                                 uint32_t *__restrict__ cell_cnt, uint32_t *__restrict__ counters) {
+    for (uint32_t base = blockIdx.x * blockDim.x; base < cap; base += gridDim.x * blockDim.x) {
+        uint32_t s = base + threadIdx.x;
+        uint32_t k = s < cap ? keys[s] : SLOSH_NONE;
+        bool occ = k != SLOSH_NONE;
+        uint64_t m = __ballot(occ);
+        uint32_t cnt = (uint32_t)__popcll(m);
+        uint32_t wbase = 0u;
+        if (lane_id() == 0 && cnt) wbase = atomicAdd(&counters[CTR_NUM_BLOCKS], cnt);
+        wbase = __shfl(wbase, 0);
+        if (occ) {
+            uint32_t hid = wbase + (uint32_t)__popcll(m & lanemask_lt());
+            vals[s] = hid;
+            hdr_key[hid] = k;
+            uint4 *cc = reinterpret_cast<uint4 *>(cell_cnt + (size_t)hid * NODES_PER_BLOCK);
+#pragma unroll
+            for (int j = 0; j < 16; j++) cc[j] = make_uint4(0u, 0u, 0u, 0u);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_count(const float4 *__restrict__ pos, uint32_t n, float h, uint32_t batched,
+                                               const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+                                               uint32_t cap, uint32_t *__restrict__ cell_cnt,
+                                               uint32_t *__restrict__ pslot, uint32_t *__restrict__ prank,
+                                               uint32_t *__restrict__ counters) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = i < n;
+    uint32_t cell = 0u, key = 0u;
+    if (valid) key = particle_key(pos[i], h, batched, &cell);
+    // Hash lookup, once per distinct block key of the wavefront.
+    uint32_t hid = SLOSH_NONE;
+    bool need = valid;
+    while (true) {
+        uint64_t m = __ballot(need);
+        if (m == 0) break;
+        int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t lk = __shfl(key, leader);
+        uint32_t hv = 0u;
+        if ((int)lane_id() == leader) hv = hash_find(keys, vals, cap, lk);
+        hv = __shfl(hv, leader);
+        if (need && key == lk) {
+            hid = hv;
+            need = false;
+        }
+    }
+    uint32_t sk = (valid && hid != SLOSH_NONE) ? hid * NODES_PER_BLOCK + cell : SLOSH_NONE;
+    if (valid && hid == SLOSH_NONE) atomicOr(&counters[CTR_ERRORS], ERR_LOOKUP);
+    // Per-(block, cell) rank: one aggregated atomic per distinct key of the wavefront.
+    bool act = sk != SLOSH_NONE;
+    uint32_t rank = 0u;
+    while (true) {
+        uint64_t m = __ballot(act);
+        if (m == 0) break;
+        int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t lk = __shfl(sk, leader);
+        bool match = act && sk == lk;
+        uint64_t mm = __ballot(match);
+        uint32_t b = 0u;
+        if ((int)lane_id() == leader) b = atomicAdd(&cell_cnt[lk], (uint32_t)__popcll(mm));
+        b = __shfl(b, leader);
+        if (match) {
+            rank = b + (uint32_t)__popcll(mm & lanemask_lt());
+            act = false;
+        }
+    }
+    if (valid) {
+        pslot[i] = sk;
+        prank[i] = rank;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_block_prepare(const uint32_t *__restrict__ cell_cnt,
+                                                       uint32_t *__restrict__ cell_start,
+                                                       uint32_t *__restrict__ blk_start, uint32_t *__restrict__ blk_cnt,
+                                                       const uint32_t *__restrict__ hdr_key,
+                                                       uint32_t *__restrict__ nbr, const uint32_t *__restrict__ keys,
+                                                       const uint32_t *__restrict__ vals,
+                                                       const uint32_t *__restrict__ prev_keys,
+                                                       const uint32_t *__restrict__ prev_vals, uint32_t cap,
+                                                       uint32_t batched, uint32_t *__restrict__ counters) {
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t waves_per_grid = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t hid = blockIdx.x * (blockDim.x >> 6) + wave; hid < B; hid += waves_per_grid) {
+        uint32_t c = cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane];
+        uint32_t x = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t y = __shfl_up(x, d);
+            if ((int)lane >= d) x += y;
+        }
+        uint32_t total = __shfl(x, 63);
+        uint32_t start = 0u;
+        if (lane == 0) start = atomicAdd(&counters[CTR_SORTED_ALLOC], total);
+        start = __shfl(start, 0);
+        cell_start[(size_t)hid * NODES_PER_BLOCK + lane] = start + x - c;
+        if (lane == 0) {
+            blk_start[hid] = start;
+            blk_cnt[hid] = total;
+        }
+        uint32_t k = hdr_key[hid];
+        int bx, by, bz;
+        uint32_t scene;
+        unpack_key(batched, k, &bx, &by, &bz, &scene);
+        uint32_t out = SLOSH_NONE;
+        if (lane < 7) {
+            const int *o = ASSOC[lane + 1];
+            out = hash_find(keys, vals, cap, pack_key(batched, scene, bx + o[0], by + o[1], bz + o[2]));
+        } else if (lane == NBR_PREV) {
+            out = hash_find(prev_keys, prev_vals, cap, k);
+        } else if (lane >= NBR_MINUS && lane < NBR_MINUS + 7) {
+            const int *o = ASSOC[lane - NBR_MINUS + 1];
+            out = hash_find(keys, vals, cap, pack_key(batched, scene, bx - o[0], by - o[1], bz - o[2]));
+        }
+        if (lane < NBR_STRIDE) nbr[(size_t)hid * NBR_STRIDE + lane] = out;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const uint32_t *__restrict__ pslot, const uint32_t *__restrict__ prank,
+                                                 const uint32_t *__restrict__ cell_start, uint32_t *__restrict__ perm,
+                                                 uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s = pslot[i];
+    if (s != SLOSH_NONE) perm[cell_start[s] + prank[i]] = i;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+
+struct GridBuffers {
+    uint32_t *keys, *vals, *prev_keys, *prev_vals;
+    uint32_t *hdr_key, *nbr, *cell_cnt, *cell_start, *blk_start, *blk_cnt;
+    uint32_t *pslot, *prank, *perm, *counters;
+};
+
+static inline uint32_t div_up(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
+static inline uint32_t min_u(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n, float h, uint32_t batched,
+                             uint32_t cap, hipStream_t stream) {
+    const uint32_t slot_grid = min_u(div_up(cap, 256), 2048);
+    const uint32_t part_grid = div_up(n, 256);
+    hipLaunchKernelGGL(k_reset_table, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, g.counters);
+    if (n > 0) hipLaunchKernelGGL(k_touch_primary, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched, g.keys,
+                                  g.vals, cap, g.counters);
+    hipLaunchKernelGGL(k_touch_neighbors, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, batched,
+                       g.counters);
+    hipLaunchKernelGGL(k_compact, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, g.hdr_key, g.cell_cnt,
+                       g.counters);
+    if (n > 0) hipLaunchKernelGGL(k_count, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched, g.keys, g.vals,
+                                  cap, g.cell_cnt, g.pslot, g.prank, g.counters);
+    const uint32_t blk_grid = min_u(div_up(cap, 4), 4096);
+    hipLaunchKernelGGL(k_block_prepare, dim3(blk_grid), dim3(256), 0, stream, g.cell_cnt, g.cell_start, g.blk_start,
+                       g.blk_cnt, g.hdr_key, g.nbr, g.keys, g.vals, g.prev_keys, g.prev_vals, cap, batched,
+                       g.counters);
+    if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(part_grid), dim3(256), 0, stream, g.pslot, g.prank, g.cell_start,
+                                  g.perm, n);
+    return hipGetLastError();
+}

@@ -1,0 +1,1076 @@
+// Particle <-> grid transfers, grid update, particle update and rigid-body
+// integration of one substep on gfx950.
+//
+//   k_p2g            one workgroup per active block: lane = associated cell,
+//                    wave w takes particles w, w+4, ... of that cell and
+//                    scatters its 27 stencil contributions into a 6x6x6 LDS
+//                    tile with LDS float atomics; the tile is written once to
+//                    HBM (no global atomics). Same per-(node, particle)
+//                    arithmetic as p2g_scatter_style.slang:84-148; only the
+//                    summation order differs.
+//   k_grid_update    one wave per block: node momentum = own tile + the halo
+//                    parts of the 7 "-1" neighbours' tiles, then
+//                    grid_update.slang:38-163 (gravity, clamp, collision
+//                    cache / collide(), boundary projection).
+//   k_g2p_update     one workgroup per block: 6x6x6 node halo in LDS, then per
+//                    primary particle g2p.slang:150-265 fused with
+//                    particle_update.slang:32-160. Outputs go to the other
+//                    particle buffer at the particle's sorted slot, so particle
+//                    storage follows the block order of the last substep.
+//   k_g2p / k_particle_update  the same two stages as separate in-place
+//                    kernels, used when a hook replaces or observes G2P.
+//   k_rigid_update   rigid_impulses.slang:70-121.
+#include "common.h"
+
+namespace {
+
+constexpr int ASSOC[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1},
+                             {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+
+// NBH_SHIFTS order of kernel.slang:120-149 (summation order of G2P).
+__constant__ uint8_t NBH[27][3] = {
+    {2, 2, 2}, {2, 0, 2}, {2, 1, 2}, {0, 2, 2}, {0, 0, 2}, {0, 1, 2}, {1, 2, 2}, {1, 0, 2}, {1, 1, 2},
+    {2, 2, 0}, {2, 0, 0}, {2, 1, 0}, {0, 2, 0}, {0, 0, 0}, {0, 1, 0}, {1, 2, 0}, {1, 0, 0}, {1, 1, 0},
+    {2, 2, 1}, {2, 0, 1}, {2, 1, 1}, {0, 2, 1}, {0, 0, 1}, {0, 1, 1}, {1, 2, 1}, {1, 0, 1}, {1, 1, 1}};
+
+// ---------------------------------------------------------------------------
+// Quadratic B-spline (kernel.slang:160-174).
+
+__device__ inline float k_eval(float x) {
+    float xa = fabsf(x);
+    float p1 = 0.75f - xa * xa;
+    float p2 = 0.5f * (1.5f - xa) * (1.5f - xa);
+    return xa < 0.5f ? p1 : (xa < 1.5f ? p2 : 0.0f);
+}
+
+__device__ inline void k_eval_all(float x, float *w) {
+    w[0] = 0.5f * (1.5f - x) * (1.5f - x);
+    w[1] = 0.75f - (x - 1.0f) * (x - 1.0f);
+    w[2] = 0.5f * (x - 0.5f) * (x - 0.5f);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 helpers on shader matrices stored row-major in float[9].
+
+__device__ inline float det3(const float *a) {
+    return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+
+__device__ inline void mul3(const float *a, const float *b, float *r) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) r[i * 3 + j] = a[i * 3 + 0] * b[0 * 3 + j] + a[i * 3 + 1] * b[1 * 3 + j] + a[i * 3 + 2] * b[2 * 3 + j];
+}
+
+// mul(transpose(a), b)
+__device__ inline void mul3_tn(const float *a, const float *b, float *r) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) r[i * 3 + j] = a[0 * 3 + i] * b[0 * 3 + j] + a[1 * 3 + i] * b[1 * 3 + j] + a[2 * 3 + i] * b[2 * 3 + j];
+}
+
+// 3x3 SVD: cyclic Jacobi eigen-decomposition of A^T A (Golub & Van Loan
+// symmetric Schur rotations), eigenvalues sorted descending, then a Givens QR
+// of A V (the layout of McAdams et al. 2011). U and V are rotations,
+// sig0 >= sig1 >= |sig2| and sign(sig2) = sign(det A). T = float or double.
+template <typename T>
+__device__ void svd3(const T *a, T *u, T *sig, T *v) {
+    const T tol = sizeof(T) == 4 ? (T)1.0e-14 : (T)1.0e-30;
+    T s[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) s[i][j] = a[0 * 3 + i] * a[0 * 3 + j] + a[1 * 3 + i] * a[1 * 3 + j] + a[2 * 3 + i] * a[2 * 3 + j];
+    T vv[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 12; sweep++) {
+        T off = s[0][1] * s[0][1] + s[0][2] * s[0][2] + s[1][2] * s[1][2];
+        T dg = s[0][0] * s[0][0] + s[1][1] * s[1][1] + s[2][2] * s[2][2];
+        if (!(off > tol * dg)) break;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            const int p = e == 2 ? 1 : 0;
+            const int q = e == 0 ? 1 : 2;
+            T apq = s[p][q];
+            if (apq == (T)0) continue;
+            T theta = (s[q][q] - s[p][p]) / ((T)2 * apq);
+            T t = (theta >= (T)0 ? (T)1 : (T)-1) / (fabs(theta) + sqrt(theta * theta + (T)1));
+            T c = (T)1 / sqrt(t * t + (T)1);
+            T sn = t * c;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                T skp = s[k][p], skq = s[k][q];
+                s[k][p] = c * skp - sn * skq;
+                s[k][q] = sn * skp + c * skq;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                T spk = s[p][k], sqk = s[q][k];
+                s[p][k] = c * spk - sn * sqk;
+                s[q][k] = sn * spk + c * sqk;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                T vkp = vv[k][p], vkq = vv[k][q];
+                vv[k][p] = c * vkp - sn * vkq;
+                vv[k][q] = sn * vkp + c * vkq;
+            }
+        }
+    }
+    T ev[3] = {s[0][0], s[1][1], s[2][2]};
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+            if (ev[i] < ev[i + 1]) {
+                T te = ev[i];
+                ev[i] = ev[i + 1];
+                ev[i + 1] = te;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    T tv = vv[k][i];
+                    vv[k][i] = -vv[k][i + 1];
+                    vv[k][i + 1] = tv;
+                }
+            }
+    T b[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) b[i][j] = a[i * 3 + 0] * vv[0][j] + a[i * 3 + 1] * vv[1][j] + a[i * 3 + 2] * vv[2][j];
+    T qm[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        const int r0 = e == 2 ? 1 : 0;
+        const int r1 = e == 0 ? 1 : 2;
+        const int col = r0;
+        T x = b[r0][col], y = b[r1][col];
+        T r = sqrt(x * x + y * y);
+        T c = (T)1, sn = (T)0;
+        if (r > (T)0) {
+            c = x / r;
+            sn = y / r;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            T b0 = b[r0][k], b1 = b[r1][k];
+            b[r0][k] = c * b0 + sn * b1;
+            b[r1][k] = -sn * b0 + c * b1;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            T q0 = qm[k][r0], q1 = qm[k][r1];
+            qm[k][r0] = c * q0 + sn * q1;
+            qm[k][r1] = -sn * q0 + c * q1;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        sig[i] = b[i][i];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            u[i * 3 + j] = qm[i][j];
+            v[i * 3 + j] = vv[i][j];
+        }
+    }
+}
+
+// U diag(s) V^T
+template <typename T>
+__device__ inline void svd_recompose(const T *u, const T *s, const T *v, T *r) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) r[i * 3 + j] = (u[i * 3 + 0] * s[0]) * v[j * 3 + 0] + (u[i * 3 + 1] * s[1]) * v[j * 3 + 1] + (u[i * 3 + 2] * s[2]) * v[j * 3 + 2];
+}
+
+// ---------------------------------------------------------------------------
+// Material models (shaders/slosh/models).
+
+// neo_hookean_elasticity.slang:16-27
+__device__ inline void nh_stress(float lambda, float mu, const float *f, float *st) {
+    float j = fmaxf(det3(f), 1.0e-10f);
+    float diag = lambda * logf(j) - mu;
+    mul3_tn(f, f, st);
+#pragma unroll
+    for (int k = 0; k < 9; k++) st[k] *= mu;
+    st[0] += diag;
+    st[4] += diag;
+    st[8] += diag;
+}
+
+// linear_elasticity.slang:135-148 (fixed-corotated)
+__device__ inline void fc_stress(float lambda, float mu, const float *f, float *st) {
+    float u[9], s[3], v[9], rc[9];
+    svd3<float>(f, u, s, v);
+    float j = s[0] * s[1] * s[2];
+    float sm1[3] = {s[0] - 1.0f, s[1] - 1.0f, s[2] - 1.0f};
+    float diag = lambda * (j - 1.0f) * j;
+    svd_recompose(u, sm1, v, rc);
+    mul3_tn(f, rc, st);
+    float k2 = 2.0f * mu;
+#pragma unroll
+    for (int k = 0; k < 9; k++) st[k] *= k2;
+    st[0] += diag;
+    st[4] += diag;
+    st[8] += diag;
+}
+
+// drucker_prager.slang:144-200: returns the projected deformation gradient in f,
+// updates state = {plastic det, hardening, log volume gain}.
+__device__ inline void dp_project(float *state, const float4 hp, float lambda, float mu, float *f) {
+    if (lambda == 0.0f) return;
+    float u[9], s[3], v[9];
+    svd3<float>(f, u, s, v);
+    float angle = hp.x + (hp.y * state[1] - hp.w) * expf(-hp.z * state[1]);
+    float s_angle = sinf(angle);
+    float alpha = sqrtf(2.0f / 3.0f) * (2.0f * s_angle) / (3.0f - s_angle);
+    const float d = 3.0f;
+    float strain[3], dev[3];
+    float lvg = state[2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) strain[i] = logf(s[i]) + lvg / d;
+    float tr = strain[0] + strain[1] + strain[2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) dev[i] = strain[i] - tr / d;
+    float ns[3];
+    float hard;
+    if (tr > 0.0f || (dev[0] == 0.0f && dev[1] == 0.0f && dev[2] == 0.0f)) {
+        ns[0] = ns[1] = ns[2] = 1.0f;
+        hard = sqrtf(strain[0] * strain[0] + strain[1] * strain[1] + strain[2] * strain[2]);
+    } else {
+        float dn = sqrtf(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+        float gamma = dn + (d * lambda + 2.0f * mu) / (2.0f * mu) * tr * alpha;
+        if (gamma <= 0.0f) return;
+#pragma unroll
+        for (int i = 0; i < 3; i++) ns[i] = expf(strain[i] - dev[i] * (gamma / dn));
+        hard = gamma;
+    }
+    float prev_det = s[0] * s[1] * s[2];
+    float new_det = ns[0] * ns[1] * ns[2];
+    state[0] = state[0] * prev_det / new_det;
+    state[2] = state[2] + logf(prev_det) - logf(new_det);
+    state[1] = state[1] + hard;
+    svd_recompose(u, ns, v, f);
+}
+
+// FP64 variants of the SVD-dependent steps: the SVD, the fixed-corotated F - R
+// assembly and the Drucker-Prager return mapping run in double and are rounded
+// once. In FP32, F - R and log(sigma) near sigma = 1 lose ~5 digits to
+// cancellation, which is the size of the parity tolerance.
+__device__ inline void fc_stress_f64(float lambda, float mu, const float *f32, float *st) {
+    double f[9], u[9], s[3], v[9], rc[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) f[k] = f32[k];
+    svd3<double>(f, u, s, v);
+    double j = s[0] * s[1] * s[2];
+    double sm1[3] = {s[0] - 1.0, s[1] - 1.0, s[2] - 1.0};
+    svd_recompose<double>(u, sm1, v, rc);
+    const double mu2 = 2.0 * (double)mu;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double r = mu2 * (f[0 * 3 + i] * rc[0 * 3 + k] + f[1 * 3 + i] * rc[1 * 3 + k] + f[2 * 3 + i] * rc[2 * 3 + k]);
+            if (i == k) r += (double)lambda * (j - 1.0) * j;
+            st[i * 3 + k] = (float)r;
+        }
+}
+
+__device__ inline void dp_project_f64(float *state, const float4 hp, float lambda_f, float mu_f, float *f32) {
+    if (lambda_f == 0.0f) return;
+    const double lambda = lambda_f, mu = mu_f;
+    double f[9], u[9], s[3], v[9], nf[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) f[k] = f32[k];
+    svd3<double>(f, u, s, v);
+    double q = state[1];
+    double angle = (double)hp.x + ((double)hp.y * q - (double)hp.w) * exp(-(double)hp.z * q);
+    double sa = sin(angle);
+    double alpha = sqrt(2.0 / 3.0) * (2.0 * sa) / (3.0 - sa);
+    const double d = 3.0;
+    double strain[3], dev[3], ns[3], hard;
+#pragma unroll
+    for (int i = 0; i < 3; i++) strain[i] = log(s[i]) + (double)state[2] / d;
+    double tr = strain[0] + strain[1] + strain[2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) dev[i] = strain[i] - tr / d;
+    if (tr > 0.0 || (dev[0] == 0.0 && dev[1] == 0.0 && dev[2] == 0.0)) {
+        ns[0] = ns[1] = ns[2] = 1.0;
+        hard = sqrt(strain[0] * strain[0] + strain[1] * strain[1] + strain[2] * strain[2]);
+    } else {
+        double dn = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+        double gamma = dn + (d * lambda + 2.0 * mu) / (2.0 * mu) * tr * alpha;
+        if (gamma <= 0.0) return;
+#pragma unroll
+        for (int i = 0; i < 3; i++) ns[i] = exp(strain[i] - dev[i] * (gamma / dn));
+        hard = gamma;
+    }
+    double prev_det = s[0] * s[1] * s[2];
+    double new_det = ns[0] * ns[1] * ns[2];
+    state[0] = (float)((double)state[0] * prev_det / new_det);
+    state[2] = (float)((double)state[2] + log(prev_det) - log(new_det));
+    state[1] = (float)((double)state[1] + hard);
+    svd_recompose<double>(u, ns, v, nf);
+#pragma unroll
+    for (int k = 0; k < 9; k++) f32[k] = (float)nf[k];
+}
+
+// ---------------------------------------------------------------------------
+// Shapes, poses and collisions (shaders/slosh/rbd, shaders/slosh/collision).
+
+__device__ inline float3 quat_rotate(float4 q, float3 v) {
+    float3 t = make_float3(2.0f * (q.y * v.z - q.z * v.y), 2.0f * (q.z * v.x - q.x * v.z), 2.0f * (q.x * v.y - q.y * v.x));
+    return make_float3(v.x + q.w * t.x + (q.y * t.z - q.z * t.y), v.y + q.w * t.y + (q.z * t.x - q.x * t.z),
+                       v.z + q.w * t.z + (q.x * t.y - q.y * t.x));
+}
+
+// Sim3 (quaternion, translation, uniform scale) point transforms.
+__device__ inline float3 sim_mul_pt(float4 rot, float4 ts, float3 p) {
+    float3 r = quat_rotate(rot, make_float3(p.x * ts.w, p.y * ts.w, p.z * ts.w));
+    return make_float3(r.x + ts.x, r.y + ts.y, r.z + ts.z);
+}
+__device__ inline float3 sim_inv_mul_pt(float4 rot, float4 ts, float3 p) {
+    float3 r = quat_rotate(make_float4(-rot.x, -rot.y, -rot.z, rot.w), make_float3(p.x - ts.x, p.y - ts.y, p.z - ts.z));
+    return make_float3(r.x / ts.w, r.y / ts.w, r.z / ts.w);
+}
+
+__device__ inline float sgn(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+// cuboid.slang:34-63
+__device__ inline bool cuboid_project_boundary(float3 he, float3 pt, float3 *out) {
+    float3 op;
+    op.x = pt.x + (fmaxf(-he.x - pt.x, 0.0f) - fmaxf(pt.x - he.x, 0.0f));
+    op.y = pt.y + (fmaxf(-he.y - pt.y, 0.0f) - fmaxf(pt.y - he.y, 0.0f));
+    op.z = pt.z + (fmaxf(-he.z - pt.z, 0.0f) - fmaxf(pt.z - he.z, 0.0f));
+    float sx = sgn(pt.x), sy = sgn(pt.y), sz = sgn(pt.z);
+    sx = sx + (fabsf(sx) - 1.0f);
+    sy = sy + (fabsf(sy) - 1.0f);
+    sz = sz + (fabsf(sz) - 1.0f);
+    float dx = he.x - sx * pt.x, dy = he.y - sy * pt.y, dz = he.z - sz * pt.z;
+    bool pick_x = dx <= dy && dx <= dz;
+    bool pick_y = dy <= dx && dy <= dz;
+    float3 ip = pt;
+    if (pick_x) ip.x = pt.x + dx * sx;
+    else if (pick_y) ip.y = pt.y + dy * sy;
+    else ip.z = pt.z + dz * sz;
+    bool inside = pt.x == op.x && pt.y == op.y && pt.z == op.z;
+    *out = inside ? ip : op;
+    return inside;
+}
+
+// ball.slang:39-50
+__device__ inline bool ball_project_boundary(float radius, float3 pt, float3 *out) {
+    float dist = sqrtf(pt.x * pt.x + pt.y * pt.y + pt.z * pt.z);
+    if (dist != 0.0f) {
+        float k = radius / dist;
+        *out = make_float3(pt.x * k, pt.y * k, pt.z * k);
+    } else {
+        *out = make_float3(0.0f, radius, 0.0f);
+    }
+    return dist <= radius;
+}
+
+struct Colliders {
+    const float4 *shapes; // 2 per body: a, b
+    const float4 *poses;  // 2 per body: rotation, translation_scale
+    const uint2 *mats;    // (type, friction bits)
+    uint32_t count;
+};
+
+struct NodeColl {
+    float4 normal_id; // normal.xyz, w = collider id (u32 bits)
+    float4 point_valid; // point.xyz, w = valid (u32 bits)
+};
+
+// collide(), collide.slang:87-154 (primitive shapes; trimesh colliders are
+// rejected at state creation).
+__device__ inline NodeColl collide(const Colliders &cs, float h, float3 point) {
+    NodeColl c;
+    c.normal_id = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+    c.point_valid = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+    float dist_cap = h * 1.5f;
+    float best = 1.0e10f;
+    for (uint32_t i = 0; i < cs.count; i++) {
+        float4 a = cs.shapes[2 * i];
+        uint32_t ty = __float_as_uint(a.w);
+        if (ty == 5u || ty == 6u) continue;
+        float4 rot = cs.poses[2 * i], ts = cs.poses[2 * i + 1];
+        float3 proj;
+        bool inside = false;
+        if (ty == 1u || ty == 0u) {
+            float3 lp = sim_inv_mul_pt(rot, ts, point);
+            float3 lproj;
+            if (ty == 1u) inside = cuboid_project_boundary(make_float3(a.x, a.y, a.z), lp, &lproj);
+            else inside = ball_project_boundary(a.x, lp, &lproj);
+            proj = sim_mul_pt(rot, ts, lproj);
+        } else {
+            proj = point; // shape.slang:200
+        }
+        float3 d = make_float3(proj.x - point.x, proj.y - point.y, proj.z - point.z);
+        if (inside || (fabsf(d.x) <= dist_cap && fabsf(d.y) <= dist_cap && fabsf(d.z) <= dist_cap)) {
+            float sign = inside ? -1.0f : 1.0f;
+            float dist = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+            if (sign * dist < best && dist != 0.0f) {
+                float k = dist * -sign;
+                c.normal_id = make_float4(d.x / k, d.y / k, d.z / k, __uint_as_float(i));
+                c.point_valid = make_float4(proj.x, proj.y, proj.z, __uint_as_float(1u));
+                best = dist;
+            }
+        }
+    }
+    return c;
+}
+
+// BoundaryCondition::project_velocity, boundary_condition.slang:31-89
+__device__ inline float3 project_velocity(uint2 bc, float3 v, float3 n, float dist, float dt) {
+    uint32_t ty = bc.x;
+    float friction = __uint_as_float(bc.y);
+    if (ty == 0u) {
+        if (dist <= 0.0f) return make_float3(0.0f, 0.0f, 0.0f);
+        return v;
+    }
+    if (ty == 1u) {
+        if (dist <= 0.0f) {
+            float nv = v.x * n.x + v.y * n.y + v.z * n.z;
+            return make_float3(v.x - n.x * nv, v.y - n.y * nv, v.z - n.z * nv);
+        }
+        return v;
+    }
+    if (ty == 2u || ty == 3u) {
+        float nv = v.x * n.x + v.y * n.y + v.z * n.z;
+        if (nv < 0.0f) {
+            if (dist <= 0.0f) {
+                float3 tv = make_float3(v.x - n.x * nv, v.y - n.y * nv, v.z - n.z * nv);
+                float tl = sqrtf(tv.x * tv.x + tv.y * tv.y + tv.z * tv.z);
+                float3 dir = make_float3(0.0f, 0.0f, 0.0f);
+                if (tl > 1.0e-8f) dir = make_float3(tv.x / tl, tv.y / tl, tv.z / tl);
+                float m = fmaxf(0.0f, tl + friction * nv);
+                return make_float3(dir.x * m, dir.y * m, dir.z * m);
+            } else if (-nv * dt > dist) {
+                float k = dist / dt + nv;
+                return make_float3(v.x - k * n.x, v.y - k * n.y, v.z - k * n.z);
+            }
+        }
+        return v;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Particle state views.
+
+struct PBuf {
+    float4 *pos;    // xyz, w = (scene << 1 | enabled) bits
+    float4 *vm;     // velocity xyz, mass
+    float *C;       // [9][pcap]
+    float *F;       // [9][pcap]
+    float *vgd;     // [pcap]
+    uint32_t *ids;  // original index
+    float *dps;     // [3][pcap] Drucker-Prager state (sand only)
+};
+
+struct PStatic {
+    const float4 *prop;  // by original id: (V0, damping, fixed bits, radius)
+    const float4 *model; // by original id: (tag bits, lambda, mu, cfl) of the elastic part
+    const float4 *dp0;   // by original id: (h0, h1, h2, h3)
+    const float4 *dp1;   // by original id: (lambda, mu, 0, 0) of the plastic part
+};
+
+// G2P of one particle from the 6x6x6 LDS halo (g2p.slang:150-265).
+__device__ inline void g2p_particle(const float4 *__restrict__ sh, float h, float3 p, bool enabled, float3 *vel,
+                                    float *vg, float *vgd_out) {
+    float4 mvm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float g[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] = 0.0f;
+    float vgd = 0.0f;
+    if (enabled) {
+        float inv_d = 4.0f / (h * h);
+        float ref[3], w[3][3];
+        float pp[3] = {p.x, p.y, p.z};
+        uint32_t lc[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            float ac = rintf(pp[i] / h) - 1.0f;
+            ref[i] = ac * h - pp[i];
+            k_eval_all(-ref[i] / h, w[i]);
+            float ab = floorf(ac / 4.0f) * 4.0f;
+            lc[i] = (uint32_t)(ac - ab);
+        }
+        uint32_t packed = lc[0] + lc[1] * 6u + lc[2] * 36u;
+#pragma unroll
+        for (int i = 0; i < 27; i++) {
+            const uint32_t sx = NBH[i][0], sy = NBH[i][1], sz = NBH[i][2];
+            float4 cd = sh[packed + sx + sy * 6u + sz * 36u];
+            float dpt[3] = {ref[0] + (float)sx * h, ref[1] + (float)sy * h, ref[2] + (float)sz * h};
+            float weight = w[0][sx] * w[1][sy] * w[2][sz];
+            mvm.x += cd.x * weight;
+            mvm.y += cd.y * weight;
+            mvm.z += cd.z * weight;
+            mvm.w += cd.w * weight;
+            float wi = weight * inv_d;
+            float cv[3] = {cd.x, cd.y, cd.z};
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) g[r * 3 + c] += wi * (cv[c] * dpt[r]);
+            vgd += weight * inv_d * (cd.x * dpt[0] + cd.y * dpt[1] + cd.z * dpt[2]);
+        }
+    }
+    *vel = make_float3(mvm.x, mvm.y, mvm.z);
+#pragma unroll
+    for (int k = 0; k < 9; k++) vg[k] = g[k];
+    *vgd_out = vgd;
+}
+
+// particle_update.slang:32-160 for one particle. `aff` holds the velocity
+// gradient on entry and the APIC affine matrix on exit.
+__device__ inline void update_particle(const StepParams &P, float3 &pos, float3 &v, float &mass, uint32_t &enabled,
+                                       float *aff, float *F, float4 prop, float4 model, const PStatic &st,
+                                       uint32_t oid, float *dps) {
+    const float dt = P.dt, h = P.h;
+    float vl = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    if (vl > h / dt) {
+        float l2 = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+        v = make_float3(v.x / l2 * h / dt, v.y / l2 * h / dt, v.z / l2 * h / dt);
+    }
+    float damp = 1.0f + prop.y * dt;
+    v = make_float3(v.x / damp, v.y / damp, v.z / damp);
+    bool fixed = __float_as_uint(prop.z) != 0u;
+    if (fixed) v = make_float3(0.0f, 0.0f, 0.0f);
+    float3 np = make_float3(pos.x + v.x * dt, pos.y + v.y * dt, pos.z + v.z * dt);
+    // F += mul(F, affine * dt)
+    float adt[9], fa[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) adt[k] = aff[k] * dt;
+    mul3(F, adt, fa);
+#pragma unroll
+    for (int k = 0; k < 9; k++) F[k] = F[k] + fa[k];
+    if (fixed) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) aff[k] = 0.0f;
+    }
+    // DefaultParticleModel::update, default.slang:51-92
+    uint32_t tag = __float_as_uint(model.x);
+    float stress[9];
+    if (tag == 1u) {
+        nh_stress(model.y, model.z, F, stress);
+    } else if (tag == 0u) {
+        if (P.svd_f64) fc_stress_f64(model.y, model.z, F, stress);
+        else fc_stress(model.y, model.z, F, stress);
+    } else if (tag == 2u || tag == 3u) {
+        float4 hp = st.dp0[oid];
+        float4 pl = st.dp1[oid];
+        if (P.svd_f64) dp_project_f64(dps, hp, pl.x, pl.y, F);
+        else dp_project(dps, hp, pl.x, pl.y, F);
+        if (tag == 3u) nh_stress(model.y, model.z, F, stress);
+        else if (P.svd_f64) fc_stress_f64(model.y, model.z, F, stress);
+        else fc_stress(model.y, model.z, F, stress);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) stress[k] = 0.0f;
+    }
+    float inv_d = 4.0f / (h * h);
+    float sk = prop.x * inv_d * dt;
+    bool ok = np.x == np.x && np.y == np.y && np.z == np.z && det3(F) > 0.0f;
+    if (ok) {
+        pos = np;
+#pragma unroll
+        for (int k = 0; k < 9; k++) aff[k] = aff[k] * mass - stress[k] * sk;
+    } else {
+        enabled = 0u;
+        v = make_float3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            F[k] = (k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f;
+            aff[k] = 0.0f;
+        }
+        mass = 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels.
+
+__global__ __launch_bounds__(256) void k_p2g(StepParams P, const float4 *__restrict__ pos, const float4 *__restrict__ vm,
+                                             const float *__restrict__ C, const float *__restrict__ fdt, uint32_t pcap,
+                                             const uint32_t *__restrict__ perm, const uint32_t *__restrict__ cell_cnt,
+                                             const uint32_t *__restrict__ cell_start,
+                                             const uint32_t *__restrict__ hdr_key, float4 *__restrict__ tiles,
+                                             const uint32_t *__restrict__ counters) {
+    __shared__ float t_x[TILE_NODES], t_y[TILE_NODES], t_z[TILE_NODES], t_m[TILE_NODES];
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+    const uint32_t tbase = lx + 6 * ly + 36 * lz;
+    const float h = P.h, inv_h = P.inv_h;
+    for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
+        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 256) {
+            t_x[j] = 0.0f;
+            t_y[j] = 0.0f;
+            t_z[j] = 0.0f;
+            t_m[j] = 0.0f;
+        }
+        __syncthreads();
+        int bx, by, bz;
+        uint32_t scene;
+        unpack_key(P.batched, hdr_key[hid], &bx, &by, &bz, &scene);
+        const uint32_t cnt = cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane];
+        const uint32_t st = cell_start[(size_t)hid * NODES_PER_BLOCK + lane];
+        float nx[3], ny[3], nz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            nx[k] = (float)(bx * 4 + lx + k) * h;
+            ny[k] = (float)(by * 4 + ly + k) * h;
+            nz[k] = (float)(bz * 4 + lz + k) * h;
+        }
+        for (uint32_t j = wave; j < cnt; j += 4) {
+            const uint32_t p = perm[st + j];
+            const float4 ps = pos[p];
+            const float4 v = vm[p];
+            float c[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) c[k] = C[(size_t)k * pcap + p];
+            float mx = v.x * v.w, my = v.y * v.w, mz = v.z * v.w;
+            if (P.has_force) {
+                mx += fdt[p];
+                my += fdt[(size_t)pcap + p];
+                mz += fdt[2 * (size_t)pcap + p];
+            }
+            float dx[3], dy[3], dz[3], wx[3], wy[3], wz[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                dx[k] = nx[k] - ps.x;
+                dy[k] = ny[k] - ps.y;
+                dz[k] = nz[k] - ps.z;
+                wx[k] = k_eval(dx[k] * inv_h);
+                wy[k] = k_eval(dy[k] * inv_h);
+                wz[k] = k_eval(dz[k] * inv_h);
+            }
+#pragma unroll
+            for (int kz = 0; kz < 3; kz++)
+#pragma unroll
+                for (int ky = 0; ky < 3; ky++)
+#pragma unroll
+                    for (int kx = 0; kx < 3; kx++) {
+                        const float w = wx[kx] * wy[ky] * wz[kz];
+                        const float ax = dx[kx] * c[0] + dy[ky] * c[3] + dz[kz] * c[6] + mx;
+                        const float ay = dx[kx] * c[1] + dy[ky] * c[4] + dz[kz] * c[7] + my;
+                        const float az = dx[kx] * c[2] + dy[ky] * c[5] + dz[kz] * c[8] + mz;
+                        const uint32_t node = tbase + kx + 6 * ky + 36 * kz;
+                        atomicAdd(&t_x[node], ax * w);
+                        atomicAdd(&t_y[node], ay * w);
+                        atomicAdd(&t_z[node], az * w);
+                        atomicAdd(&t_m[node], v.w * w);
+                    }
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 256)
+            tiles[(size_t)hid * TILE_NODES + j] = make_float4(t_x[j], t_y[j], t_z[j], t_m[j]);
+        __syncthreads();
+    }
+}
+
+// mode 0: node momentum from tiles; mode 1: nodes already hold momentum (P2G hook).
+// do_update 0: only combine tiles (grid update disabled).
+__global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 *__restrict__ tiles,
+                                                     float4 *__restrict__ nodes, const uint32_t *__restrict__ nbr,
+                                                     const uint32_t *__restrict__ hdr_key, Colliders cs,
+                                                     const NodeColl *__restrict__ coll_prev, NodeColl *__restrict__ coll_cur,
+                                                     const uint32_t *__restrict__ counters, uint32_t mode,
+                                                     uint32_t do_update) {
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const int x = lane & 3, y = (lane >> 2) & 3, z = lane >> 4;
+    const uint32_t waves_per_grid = gridDim.x * 4;
+    for (uint32_t hid = blockIdx.x * 4 + wave; hid < B; hid += waves_per_grid) {
+        float4 acc;
+        if (mode == 0) {
+            acc = tiles[(size_t)hid * TILE_NODES + x + 6 * y + 36 * z];
+#pragma unroll
+            for (int d = 1; d < 8; d++) {
+                const int ox = ASSOC[d][0], oy = ASSOC[d][1], oz = ASSOC[d][2];
+                if ((ox == 0 || x <= 1) && (oy == 0 || y <= 1) && (oz == 0 || z <= 1)) {
+                    uint32_t src = nbr[(size_t)hid * NBR_STRIDE + NBR_MINUS + d - 1];
+                    if (src != SLOSH_NONE) {
+                        float4 t = tiles[(size_t)src * TILE_NODES + (x + 4 * ox) + 6 * (y + 4 * oy) + 36 * (z + 4 * oz)];
+                        acc.x += t.x;
+                        acc.y += t.y;
+                        acc.z += t.z;
+                        acc.w += t.w;
+                    }
+                }
+            }
+        } else {
+            acc = nodes[(size_t)hid * NODES_PER_BLOCK + lane];
+        }
+        if (!do_update) {
+            nodes[(size_t)hid * NODES_PER_BLOCK + lane] = acc;
+            continue;
+        }
+        int bx, by, bz;
+        uint32_t scene;
+        unpack_key(P.batched, hdr_key[hid], &bx, &by, &bz, &scene);
+        const float h = P.h, dt = P.dt;
+        float3 cell_pos = make_float3((float)(bx * 4 + x) * h, (float)(by * 4 + y) * h, (float)(bz * 4 + z) * h);
+        // update_single_cell, grid_update.slang:154-163
+        const float mass = acc.w;
+        const float inv_mass = mass > 0.0f ? 1.0f / mass : 0.0f;
+        const float lim = h / dt;
+        float3 v;
+        v.x = fminf(fmaxf((acc.x + mass * P.gravity[0] * dt) * inv_mass, -lim), lim);
+        v.y = fminf(fmaxf((acc.y + mass * P.gravity[1] * dt) * inv_mass, -lim), lim);
+        v.z = fminf(fmaxf((acc.z + mass * P.gravity[2] * dt) * inv_mass, -lim), lim);
+        if (cs.count > 0) {
+            NodeColl c;
+            bool have = true;
+            if (P.coll_cache) {
+                uint32_t prev = nbr[(size_t)hid * NBR_STRIDE + NBR_PREV];
+                if (prev != SLOSH_NONE && coll_prev != nullptr) c = coll_prev[(size_t)prev * NODES_PER_BLOCK + lane];
+                else c = collide(cs, h, cell_pos);
+                coll_cur[(size_t)hid * NODES_PER_BLOCK + lane] = c;
+            } else if (mass > 0.0f) {
+                // Static colliders: the cached record equals a fresh collide(); a
+                // zero-mass node has v = 0, which every boundary condition keeps.
+                c = collide(cs, h, cell_pos);
+            } else {
+                have = false;
+            }
+            if (have && __float_as_uint(c.point_valid.w) != 0u) {
+                uint2 bc = cs.mats[__float_as_uint(c.normal_id.w)];
+                float3 n = make_float3(c.normal_id.x, c.normal_id.y, c.normal_id.z);
+                float3 dd = make_float3(cell_pos.x - c.point_valid.x, cell_pos.y - c.point_valid.y,
+                                        cell_pos.z - c.point_valid.z);
+                float dist = (dd.x * n.x + dd.y * n.y + dd.z * n.z) - h;
+                v = project_velocity(bc, v, n, dist, dt);
+            }
+        }
+        nodes[(size_t)hid * NODES_PER_BLOCK + lane] = make_float4(v.x, v.y, v.z, mass);
+    }
+}
+
+__device__ inline void load_halo(float4 *__restrict__ sh, const float4 *__restrict__ nodes,
+                                 const uint32_t *__restrict__ nbr, uint32_t hid) {
+    for (uint32_t j = threadIdx.x; j < TILE_NODES; j += blockDim.x) {
+        uint32_t sx = j % 6u, sy = (j / 6u) % 6u, sz = j / 36u;
+        uint32_t ox = sx >= 4u, oy = sy >= 4u, oz = sz >= 4u;
+        uint32_t oi = ox * 4u + oy * 2u + oz;
+        uint32_t src = oi == 0u ? hid : nbr[(size_t)hid * NBR_STRIDE + oi - 1u];
+        uint32_t node = (sx - 4u * ox) + 4u * (sy - 4u * oy) + 16u * (sz - 4u * oz);
+        sh[j] = src != SLOSH_NONE ? nodes[(size_t)src * NODES_PER_BLOCK + node] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
+                                                    float *__restrict__ fdt, const uint32_t *__restrict__ perm,
+                                                    const uint32_t *__restrict__ blk_start,
+                                                    const uint32_t *__restrict__ blk_cnt,
+                                                    const uint32_t *__restrict__ nbr, const float4 *__restrict__ nodes,
+                                                    const uint32_t *__restrict__ counters) {
+    __shared__ float4 sh[TILE_NODES];
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
+        load_halo(sh, nodes, nbr, hid);
+        __syncthreads();
+        const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
+        for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+            const uint32_t slot = s0 + j;
+            const uint32_t p = perm[slot];
+            const float4 ps = in.pos[p];
+            const uint32_t wbits = __float_as_uint(ps.w);
+            uint32_t enabled = wbits & 1u;
+            float3 pos = make_float3(ps.x, ps.y, ps.z);
+            float3 v;
+            float aff[9], F[9];
+            float vgd;
+            g2p_particle(sh, P.h, pos, enabled != 0u, &v, aff, &vgd);
+            float mass = in.vm[p].w;
+#pragma unroll
+            for (int k = 0; k < 9; k++) F[k] = in.F[(size_t)k * pcap + p];
+            const uint32_t oid = in.ids[p];
+            const float4 prop = st.prop[oid];
+            const float4 model = st.model[oid];
+            float dps[3] = {0.0f, 0.0f, 0.0f};
+            if (P.has_sand) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) dps[k] = in.dps[(size_t)k * pcap + p];
+            }
+            update_particle(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
+            out.pos[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
+            out.vm[slot] = make_float4(v.x, v.y, v.z, mass);
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                out.C[(size_t)k * pcap + slot] = aff[k];
+                out.F[(size_t)k * pcap + slot] = F[k];
+            }
+            out.vgd[slot] = vgd;
+            out.ids[slot] = oid;
+            if (P.has_sand) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) out.dps[(size_t)k * pcap + slot] = dps[k];
+            }
+            if (P.has_force) {
+                fdt[slot] = 0.0f;
+                fdt[(size_t)pcap + slot] = 0.0f;
+                fdt[2 * (size_t)pcap + slot] = 0.0f;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// G2P only, in place (hook path): writes velocity, velocity gradient, vel_grad_det.
+__global__ __launch_bounds__(256) void k_g2p(StepParams P, PBuf buf, uint32_t pcap, const uint32_t *__restrict__ perm,
+                                             const uint32_t *__restrict__ blk_start,
+                                             const uint32_t *__restrict__ blk_cnt, const uint32_t *__restrict__ nbr,
+                                             const float4 *__restrict__ nodes, const uint32_t *__restrict__ counters) {
+    __shared__ float4 sh[TILE_NODES];
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
+        load_halo(sh, nodes, nbr, hid);
+        __syncthreads();
+        const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
+        for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+            const uint32_t p = perm[s0 + j];
+            const float4 ps = buf.pos[p];
+            float3 v;
+            float g[9];
+            float vgd;
+            g2p_particle(sh, P.h, make_float3(ps.x, ps.y, ps.z), (__float_as_uint(ps.w) & 1u) != 0u, &v, g, &vgd);
+            float4 o = buf.vm[p];
+            buf.vm[p] = make_float4(v.x, v.y, v.z, o.w);
+#pragma unroll
+            for (int k = 0; k < 9; k++) buf.C[(size_t)k * pcap + p] = g[k];
+            buf.vgd[p] = vgd;
+        }
+        __syncthreads();
+    }
+}
+
+// Particle update only, in place (hook path), over every particle.
+__global__ __launch_bounds__(256) void k_particle_update(StepParams P, PBuf buf, PStatic st, uint32_t pcap,
+                                                         float *__restrict__ fdt) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    const float4 ps = buf.pos[p];
+    const uint32_t wbits = __float_as_uint(ps.w);
+    uint32_t enabled = wbits & 1u;
+    float3 pos = make_float3(ps.x, ps.y, ps.z);
+    float4 vmv = buf.vm[p];
+    float3 v = make_float3(vmv.x, vmv.y, vmv.z);
+    float mass = vmv.w;
+    float aff[9], F[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        aff[k] = buf.C[(size_t)k * pcap + p];
+        F[k] = buf.F[(size_t)k * pcap + p];
+    }
+    const uint32_t oid = buf.ids[p];
+    float dps[3] = {0.0f, 0.0f, 0.0f};
+    if (P.has_sand) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) dps[k] = buf.dps[(size_t)k * pcap + p];
+    }
+    update_particle(P, pos, v, mass, enabled, aff, F, st.prop[oid], st.model[oid], st, oid, dps);
+    buf.pos[p] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
+    buf.vm[p] = make_float4(v.x, v.y, v.z, mass);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        buf.C[(size_t)k * pcap + p] = aff[k];
+        buf.F[(size_t)k * pcap + p] = F[k];
+    }
+    if (P.has_sand) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) buf.dps[(size_t)k * pcap + p] = dps[k];
+    }
+    if (P.has_force) {
+        fdt[p] = 0.0f;
+        fdt[(size_t)pcap + p] = 0.0f;
+        fdt[2 * (size_t)pcap + p] = 0.0f;
+    }
+}
+
+struct Bodies {
+    float4 *poses;        // 2 per body
+    float4 *vels;         // 2 per body: linear, angular
+    const float4 *mprops; // 5 per body: inv_inertia rows 0..2, inv_mass, com (world)
+    const float4 *local_com; // 1 per body
+    int *impulses;        // 6 per body: linear xyz, angular xyz (fixed point x 1e5)
+    uint32_t count;
+};
+
+__device__ inline float len3(float3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+// rigid_impulses.slang:70-121 with body.slang:63-130
+__global__ void k_rigid_update(StepParams P, Bodies b) {
+    const uint32_t id = threadIdx.x;
+    if (id >= b.count) return;
+    const float dt = P.dt;
+    float3 lin, ang;
+    lin.x = (float)b.impulses[6 * id + 0] / 1.0e5f;
+    lin.y = (float)b.impulses[6 * id + 1] / 1.0e5f;
+    lin.z = (float)b.impulses[6 * id + 2] / 1.0e5f;
+    ang.x = (float)b.impulses[6 * id + 3] / 1.0e5f;
+    ang.y = (float)b.impulses[6 * id + 4] / 1.0e5f;
+    ang.z = (float)b.impulses[6 * id + 5] / 1.0e5f;
+#pragma unroll
+    for (int k = 0; k < 6; k++) b.impulses[6 * id + k] = 0;
+    const float4 ii0 = b.mprops[5 * id + 0], ii1 = b.mprops[5 * id + 1], ii2 = b.mprops[5 * id + 2];
+    const float4 im = b.mprops[5 * id + 3];
+    float4 vl = b.vels[2 * id], va = b.vels[2 * id + 1];
+    float3 nl = make_float3(vl.x + im.x * lin.x, vl.y + im.y * lin.y, vl.z + im.z * lin.z);
+    float3 na = make_float3(va.x + (ang.x * ii0.x + ang.y * ii1.x + ang.z * ii2.x),
+                            va.y + (ang.x * ii0.y + ang.y * ii1.y + ang.z * ii2.y),
+                            va.z + (ang.x * ii0.z + ang.y * ii1.z + ang.z * ii2.z));
+    float ln = len3(nl), an = len3(na);
+    float lin_limit = 0.1f * P.h / dt;
+    if (len3(lin) != 0.0f || len3(ang) != 0.0f) {
+        if (ln > lin_limit) nl = make_float3(nl.x * (lin_limit / ln), nl.y * (lin_limit / ln), nl.z * (lin_limit / ln));
+        if (an > 1.0f) na = make_float3(na.x * (1.0f / an), na.y * (1.0f / an), na.z * (1.0f / an));
+    }
+    // integrate_velocity
+    float4 rot = b.poses[2 * id], ts = b.poses[2 * id + 1];
+    float4 lc = b.local_com[id];
+    float3 init_com = sim_mul_pt(rot, ts, make_float3(lc.x, lc.y, lc.z));
+    float3 sa = make_float3(na.x * dt, na.y * dt, na.z * dt);
+    float angle = len3(sa);
+    float4 dq = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    if (angle != 0.0f) {
+        float s = sinf(angle * 0.5f) / angle;
+        dq = make_float4(sa.x * s, sa.y * s, sa.z * s, cosf(angle * 0.5f));
+    }
+    float3 rr = quat_rotate(dq, make_float3(ts.x - init_com.x, ts.y - init_com.y, ts.z - init_com.z));
+    float4 nq;
+    nq.x = dq.w * rot.x + dq.x * rot.w + dq.y * rot.z - dq.z * rot.y;
+    nq.y = dq.w * rot.y - dq.x * rot.z + dq.y * rot.w + dq.z * rot.x;
+    nq.z = dq.w * rot.z + dq.x * rot.y - dq.y * rot.x + dq.z * rot.w;
+    nq.w = dq.w * rot.w - dq.x * rot.x - dq.y * rot.y - dq.z * rot.z;
+    float sq = nq.x * nq.x + nq.y * nq.y + nq.z * nq.z + nq.w * nq.w;
+    float k = 0.5f * (3.0f - sq);
+    b.poses[2 * id] = make_float4(nq.x * k, nq.y * k, nq.z * k, nq.w * k);
+    b.poses[2 * id + 1] = make_float4(init_com.x + rr.x * ts.w + nl.x * dt, init_com.y + rr.y * ts.w + nl.y * dt,
+                                      init_com.z + rr.z * ts.w + nl.z * dt, ts.w);
+    nl.x += P.gravity[0] * (im.x != 0.0f ? 1.0f : 0.0f) * dt;
+    nl.y += P.gravity[1] * (im.y != 0.0f ? 1.0f : 0.0f) * dt;
+    nl.z += P.gravity[2] * (im.z != 0.0f ? 1.0f : 0.0f) * dt;
+    b.vels[2 * id] = make_float4(nl.x, nl.y, nl.z, 0.0f);
+    b.vels[2 * id + 1] = make_float4(na.x, na.y, na.z, 0.0f);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers (declarations in slosh_api.hip).
+
+struct TransferBuffers {
+    const uint32_t *perm, *cell_cnt, *cell_start, *blk_start, *blk_cnt, *hdr_key, *nbr, *counters;
+    float4 *tiles, *nodes;
+    void *coll_prev, *coll_cur;
+    float *fdt;
+};
+
+struct ParticleBuffers {
+    float4 *pos, *vm;
+    float *C, *F, *vgd, *dps;
+    uint32_t *ids;
+};
+
+struct StaticBuffers {
+    const float4 *prop, *model, *dp0, *dp1;
+};
+
+struct ColliderBuffers {
+    const float4 *shapes, *poses;
+    const uint2 *mats;
+    uint32_t count;
+};
+
+struct BodyBuffers {
+    float4 *poses, *vels;
+    const float4 *mprops, *local_com;
+    int *impulses;
+    uint32_t count;
+};
+
+static inline uint32_t div_up_t(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
+static inline uint32_t min_t(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+static inline PBuf to_pbuf(const ParticleBuffers &b) {
+    PBuf p;
+    p.pos = b.pos;
+    p.vm = b.vm;
+    p.C = b.C;
+    p.F = b.F;
+    p.vgd = b.vgd;
+    p.ids = b.ids;
+    p.dps = b.dps;
+    return p;
+}
+
+uint32_t block_grid_size(uint32_t cap, uint32_t per_wg) { return min_t(div_up_t(cap, per_wg), 8192); }
+
+hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t pcap, const TransferBuffers &t,
+                      hipStream_t stream) {
+    hipLaunchKernelGGL(k_p2g, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, pb.pos, pb.vm, pb.C, t.fdt, pcap,
+                       t.perm, t.cell_cnt, t.cell_start, t.hdr_key, t.tiles, t.counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_update(const StepParams &P, const TransferBuffers &t, const ColliderBuffers &c, uint32_t mode,
+                              uint32_t do_update, hipStream_t stream) {
+    Colliders cs;
+    cs.shapes = c.shapes;
+    cs.poses = c.poses;
+    cs.mats = c.mats;
+    cs.count = c.count;
+    hipLaunchKernelGGL(k_grid_update, dim3(block_grid_size(P.cap, 4)), dim3(256), 0, stream, P, t.tiles, t.nodes, t.nbr,
+                       t.hdr_key, cs, (const NodeColl *)t.coll_prev, (NodeColl *)t.coll_cur, t.counters, mode,
+                       do_update);
+    return hipGetLastError();
+}
+
+hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, const ParticleBuffers &out,
+                             const StaticBuffers &s, uint32_t pcap, const TransferBuffers &t, hipStream_t stream) {
+    PStatic st;
+    st.prop = s.prop;
+    st.model = s.model;
+    st.dp0 = s.dp0;
+    st.dp1 = s.dp1;
+    hipLaunchKernelGGL(k_g2p_update, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(in),
+                       to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, t.nodes, t.counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_g2p(const StepParams &P, const ParticleBuffers &b, uint32_t pcap, const TransferBuffers &t,
+                      hipStream_t stream) {
+    hipLaunchKernelGGL(k_g2p, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(b), pcap, t.perm,
+                       t.blk_start, t.blk_cnt, t.nbr, t.nodes, t.counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
+                                  float *fdt, hipStream_t stream) {
+    if (P.n == 0) return hipSuccess;
+    PStatic st;
+    st.prop = s.prop;
+    st.model = s.model;
+    st.dp0 = s.dp0;
+    st.dp1 = s.dp1;
+    hipLaunchKernelGGL(k_particle_update, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st, pcap, fdt);
+    return hipGetLastError();
+}
+
+hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, hipStream_t stream) {
+    if (bb.count == 0) return hipSuccess;
+    Bodies b;
+    b.poses = bb.poses;
+    b.vels = bb.vels;
+    b.mprops = bb.mprops;
+    b.local_com = bb.local_com;
+    b.impulses = bb.impulses;
+    b.count = bb.count;
+    hipLaunchKernelGGL(k_rigid_update, dim3(1), dim3(64), 0, stream, P, b);
+    return hipGetLastError();
+}

@@ -1,0 +1,1379 @@
+// Host implementation of the C ABI declared in include/slosh_hip.h.
+//
+// Device state layout (one slosh_data):
+//   particles, two buffers (current / next), indexed by storage slot:
+//     pos   float4  xyz, w = (scene << 1 | enabled) as u32 bits
+//     vm    float4  velocity xyz, mass
+//     C     float[9][pcap]  APIC affine matrix (velocity gradient between G2P
+//                           and the particle update), shader matrix row-major
+//     F     float[9][pcap]  deformation gradient
+//     vgd   float[pcap]     vel_grad_det
+//     ids   u32[pcap]       original particle index of the slot
+//     dps   float[3][pcap]  Drucker-Prager state (allocated iff a sand particle exists)
+//   force_dt float[3][pcap] by current slot (zero after every particle update)
+//   static records by original index: prop (V0, damping, fixed, radius),
+//     model (tag, lambda, mu, cfl of the elastic part), dp0 (h0..h3), dp1 (lambda, mu)
+//   grid: two hash tables (keys/vals, ping-pong = previous substep), per block
+//     key, neighbour table, 64 cell counts/starts, particle range, a 6x6x6
+//     float4 P2G tile, 64 float4 nodes, two collision caches (moving colliders).
+// The fused G2P/particle-update kernel writes the next buffer in sorted order,
+// so storage follows the block order of the previous substep; readback
+// restores the original order through `ids`.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/slosh_hip.h"
+#include "common.h"
+
+// ---- launchers from the kernel translation units -------------------------
+struct GridBuffers {
+    uint32_t *keys, *vals, *prev_keys, *prev_vals;
+    uint32_t *hdr_key, *nbr, *cell_cnt, *cell_start, *blk_start, *blk_cnt;
+    uint32_t *pslot, *prank, *perm, *counters;
+};
+hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n, float h, uint32_t batched,
+                             uint32_t cap, hipStream_t stream);
+struct TransferBuffers {
+    const uint32_t *perm, *cell_cnt, *cell_start, *blk_start, *blk_cnt, *hdr_key, *nbr, *counters;
+    float4 *tiles, *nodes;
+    void *coll_prev, *coll_cur;
+    float *fdt;
+};
+struct ParticleBuffers {
+    float4 *pos, *vm;
+    float *C, *F, *vgd, *dps;
+    uint32_t *ids;
+};
+struct StaticBuffers {
+    const float4 *prop, *model, *dp0, *dp1;
+};
+struct ColliderBuffers {
+    const float4 *shapes, *poses;
+    const uint2 *mats;
+    uint32_t count;
+};
+struct BodyBuffers {
+    float4 *poses, *vels;
+    const float4 *mprops, *local_com;
+    int *impulses;
+    uint32_t count;
+};
+hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t pcap, const TransferBuffers &t,
+                      hipStream_t stream);
+hipError_t launch_grid_update(const StepParams &P, const TransferBuffers &t, const ColliderBuffers &c, uint32_t mode,
+                              uint32_t do_update, hipStream_t stream);
+hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, const ParticleBuffers &out,
+                             const StaticBuffers &s, uint32_t pcap, const TransferBuffers &t, hipStream_t stream);
+hipError_t launch_g2p(const StepParams &P, const ParticleBuffers &b, uint32_t pcap, const TransferBuffers &t,
+                      hipStream_t stream);
+hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
+                                  float *fdt, hipStream_t stream);
+hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, hipStream_t stream);
+
+// ---- error plumbing ---------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess) return fail(SLOSH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+static int dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+    if (e != hipSuccess) return fail(SLOSH_E_NO_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return SLOSH_OK;
+}
+
+template <typename T>
+static void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+#define TRY(expr)                    \
+    do {                             \
+        int r_ = (expr);             \
+        if (r_ != SLOSH_OK) return r_; \
+    } while (0)
+
+// ---- objects ----------------------------------------------------------------
+struct slosh_pipeline {
+    slosh_kernels kernels;
+};
+
+struct PBufs {
+    float4 *pos = nullptr, *vm = nullptr;
+    float *C = nullptr, *F = nullptr, *vgd = nullptr, *dps = nullptr;
+    uint32_t *ids = nullptr;
+};
+
+struct slosh_data {
+    slosh_pipeline *pipeline = nullptr;
+    size_t n = 0, pcap = 0;
+    uint32_t cap = 0;
+    float h = 0.0f;
+    slosh_sim_params params{};
+    uint32_t batched = 0;
+    uint32_t has_sand = 0;
+    uint32_t has_force = 0;
+    uint32_t coll_cache = 0;
+    uint32_t coll_prev_valid = 0;
+    uint32_t svd_f64 = 1;
+    int cur_p = 0;  // current particle buffer
+    int cur_g = 0;  // hash table / collision cache written by the last substep
+    PBufs pb[2];
+    float *fdt = nullptr;
+    float4 *s_prop = nullptr, *s_model = nullptr, *s_dp0 = nullptr, *s_dp1 = nullptr;
+    std::vector<slosh_particle_model> models_host;      // by original index
+    std::vector<slosh_particle_properties> props_host;  // by original index
+    std::vector<uint32_t> scene_host;                   // by original index
+    // grid
+    uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
+    uint32_t *hdr_key = nullptr, *nbr = nullptr, *cell_cnt = nullptr, *cell_start = nullptr;
+    uint32_t *blk_start = nullptr, *blk_cnt = nullptr, *pslot = nullptr, *prank = nullptr, *perm = nullptr;
+    uint32_t *counters = nullptr;
+    float4 *tiles = nullptr, *nodes = nullptr;
+    float4 *coll[2] = {nullptr, nullptr};
+    // bodies
+    uint32_t nbodies = 0;
+    std::vector<slosh_body_desc> bodies_host;
+    float4 *shapes = nullptr, *poses = nullptr, *vels = nullptr, *mprops = nullptr, *local_com = nullptr;
+    uint2 *mats = nullptr;
+    int *impulses = nullptr;
+    // captured substep pairs, one per starting parity
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t capture_stream = nullptr;
+};
+
+struct slosh_timestamps {
+    struct Rec {
+        int pass;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double acc[SLOSH_NUM_PASSES] = {0};
+    uint32_t count[SLOSH_NUM_PASSES] = {0};
+};
+
+enum { PASS_GRID_SORT = 0, PASS_P2G, PASS_GRID_UPDATE, PASS_G2P, PASS_PARTICLES_UPDATE, PASS_INTEGRATE_BODIES };
+
+// ---- helpers ----------------------------------------------------------------
+static uint32_t next_pow2(uint32_t v) {
+    uint32_t c = 1;
+    while (c < v) c <<= 1;
+    return c;
+}
+
+static float u2f(uint32_t x) {
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
+static uint32_t f2u(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    return x;
+}
+
+static void invalidate_graphs(slosh_data *d) {
+    for (int i = 0; i < 4; i++)
+        if (d->graph[i]) {
+            (void)hipGraphExecDestroy(d->graph[i]);
+            d->graph[i] = nullptr;
+        }
+}
+
+static int alloc_particle_buffers(slosh_data *d, size_t pcap, bool sand) {
+    for (int b = 0; b < 2; b++) {
+        PBufs &p = d->pb[b];
+        TRY(dalloc(&p.pos, pcap));
+        TRY(dalloc(&p.vm, pcap));
+        TRY(dalloc(&p.C, 9 * pcap));
+        TRY(dalloc(&p.F, 9 * pcap));
+        TRY(dalloc(&p.vgd, pcap));
+        TRY(dalloc(&p.ids, pcap));
+        if (sand) TRY(dalloc(&p.dps, 3 * pcap));
+    }
+    TRY(dalloc(&d->fdt, 3 * pcap));
+    TRY(dalloc(&d->s_prop, pcap));
+    TRY(dalloc(&d->s_model, pcap));
+    if (sand) {
+        TRY(dalloc(&d->s_dp0, pcap));
+        TRY(dalloc(&d->s_dp1, pcap));
+    }
+    TRY(dalloc(&d->pslot, pcap));
+    TRY(dalloc(&d->prank, pcap));
+    TRY(dalloc(&d->perm, pcap));
+    HIP_TRY(hipMemset(d->fdt, 0, sizeof(float) * 3 * (pcap ? pcap : 1)));
+    return SLOSH_OK;
+}
+
+static void free_particle_buffers(slosh_data *d) {
+    for (int b = 0; b < 2; b++) {
+        PBufs &p = d->pb[b];
+        dfree(p.pos);
+        dfree(p.vm);
+        dfree(p.C);
+        dfree(p.F);
+        dfree(p.vgd);
+        dfree(p.ids);
+        dfree(p.dps);
+    }
+    dfree(d->fdt);
+    dfree(d->s_prop);
+    dfree(d->s_model);
+    dfree(d->s_dp0);
+    dfree(d->s_dp1);
+    dfree(d->pslot);
+    dfree(d->prank);
+    dfree(d->perm);
+}
+
+static bool model_is_sand(const slosh_particle_model &m) { return m.words[0] == 2u || m.words[0] == 3u; }
+
+// Host-side staging of particles in the internal layout (SoA, slot order).
+struct Staging {
+    size_t n = 0;
+    std::vector<float> pos, vm, C, F, vgd, dps, fdt;
+    std::vector<uint32_t> ids;
+    std::vector<float> prop, model, dp0, dp1;  // 4 floats per particle (by original index)
+    void resize(size_t count) {
+        n = count;
+        pos.assign(4 * count, 0.f);
+        vm.assign(4 * count, 0.f);
+        C.assign(9 * count, 0.f);
+        F.assign(9 * count, 0.f);
+        vgd.assign(count, 0.f);
+        dps.assign(3 * count, 0.f);
+        fdt.assign(3 * count, 0.f);
+        ids.assign(count, 0u);
+        prop.assign(4 * count, 0.f);
+        model.assign(4 * count, 0.f);
+        dp0.assign(4 * count, 0.f);
+        dp1.assign(4 * count, 0.f);
+    }
+};
+
+static int stage_particles(const slosh_particles *p, uint32_t id_base, Staging &s, bool *any_force, bool *any_sand) {
+    const size_t n = p->len;
+    s.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        const slosh_kinematics &k = p->kinematics[i];
+        const slosh_particle_model &m = p->models[i];
+        const slosh_particle_properties &pr = p->properties[i];
+        uint32_t scene = p->scene_ids ? p->scene_ids[i] : 0u;
+        if (scene >= SLOSH_MAX_SCENES) return fail(SLOSH_E_INVALID, "scene id out of range");
+        s.pos[4 * i + 0] = p->positions[4 * i + 0];
+        s.pos[4 * i + 1] = p->positions[4 * i + 1];
+        s.pos[4 * i + 2] = p->positions[4 * i + 2];
+        s.pos[4 * i + 3] = u2f((scene << 1) | (k.enabled != 0u ? 1u : 0u));
+        for (int c = 0; c < 3; c++) s.vm[4 * i + c] = k.velocity[c];
+        s.vm[4 * i + 3] = k.mass;
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                s.C[(size_t)(r * 3 + c) * n + i] = k.affine.m[r][c];
+                s.F[(size_t)(r * 3 + c) * n + i] = p->def_grad[i].m[r][c];
+            }
+        s.vgd[i] = k.vel_grad_det;
+        s.ids[i] = id_base + (uint32_t)i;
+        for (int c = 0; c < 3; c++) {
+            s.fdt[(size_t)c * n + i] = k.force_dt[c];
+            if (k.force_dt[c] != 0.0f) *any_force = true;
+        }
+        s.prop[4 * i + 0] = pr.init_volume;
+        s.prop[4 * i + 1] = pr.damping;
+        s.prop[4 * i + 2] = u2f(pr.fixed);
+        s.prop[4 * i + 3] = pr.init_radius;
+        const uint32_t tag = m.words[0];
+        s.model[4 * i + 0] = u2f(tag);
+        if (tag == 0u || tag == 1u) {
+            for (int c = 0; c < 3; c++) s.model[4 * i + 1 + c] = u2f(m.words[1 + c]);
+        } else if (tag == 2u || tag == 3u) {
+            *any_sand = true;
+            for (int c = 0; c < 3; c++) {
+                s.model[4 * i + 1 + c] = u2f(m.words[10 + c]);
+                s.dps[(size_t)c * n + i] = u2f(m.words[1 + c]);
+            }
+            for (int c = 0; c < 4; c++) s.dp0[4 * i + c] = u2f(m.words[4 + c]);
+            s.dp1[4 * i + 0] = u2f(m.words[8]);
+            s.dp1[4 * i + 1] = u2f(m.words[9]);
+        }
+    }
+    return SLOSH_OK;
+}
+
+// Uploads staged particles into slots [slot0, slot0+n) of buffer `b` (ids starting at id0).
+static int upload_staging(slosh_data *d, const Staging &s, int b, size_t slot0, size_t id0, hipStream_t st) {
+    const size_t n = s.n;
+    if (n == 0) return SLOSH_OK;
+    PBufs &p = d->pb[b];
+    const size_t pc = d->pcap;
+    HIP_TRY(hipMemcpyAsync(p.pos + slot0, s.pos.data(), 16 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p.vm + slot0, s.vm.data(), 16 * n, hipMemcpyHostToDevice, st));
+    for (int k = 0; k < 9; k++) {
+        HIP_TRY(hipMemcpyAsync(p.C + k * pc + slot0, s.C.data() + k * n, 4 * n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(p.F + k * pc + slot0, s.F.data() + k * n, 4 * n, hipMemcpyHostToDevice, st));
+    }
+    HIP_TRY(hipMemcpyAsync(p.vgd + slot0, s.vgd.data(), 4 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p.ids + slot0, s.ids.data(), 4 * n, hipMemcpyHostToDevice, st));
+    if (p.dps)
+        for (int k = 0; k < 3; k++)
+            HIP_TRY(hipMemcpyAsync(p.dps + k * pc + slot0, s.dps.data() + k * n, 4 * n, hipMemcpyHostToDevice, st));
+    for (int k = 0; k < 3; k++)
+        HIP_TRY(hipMemcpyAsync(d->fdt + k * pc + slot0, s.fdt.data() + k * n, 4 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d->s_prop + id0, s.prop.data(), 16 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d->s_model + id0, s.model.data(), 16 * n, hipMemcpyHostToDevice, st));
+    if (d->s_dp0) {
+        HIP_TRY(hipMemcpyAsync(d->s_dp0 + id0, s.dp0.data(), 16 * n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d->s_dp1 + id0, s.dp1.data(), 16 * n, hipMemcpyHostToDevice, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return SLOSH_OK;
+}
+
+// Downloads the current particle buffer (slot order).
+struct Snapshot {
+    std::vector<float> pos, vm, C, F, vgd, dps, fdt;
+    std::vector<uint32_t> ids;
+};
+
+static int download_current(slosh_data *d, hipStream_t st, Snapshot &s) {
+    const size_t n = d->n, pc = d->pcap;
+    PBufs &p = d->pb[d->cur_p];
+    s.pos.resize(4 * n);
+    s.vm.resize(4 * n);
+    s.C.resize(9 * n);
+    s.F.resize(9 * n);
+    s.vgd.resize(n);
+    s.ids.resize(n);
+    s.fdt.resize(3 * n);
+    s.dps.assign(3 * n, 0.f);
+    if (n == 0) return SLOSH_OK;
+    HIP_TRY(hipMemcpyAsync(s.pos.data(), p.pos, 16 * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s.vm.data(), p.vm, 16 * n, hipMemcpyDeviceToHost, st));
+    for (int k = 0; k < 9; k++) {
+        HIP_TRY(hipMemcpyAsync(s.C.data() + k * n, p.C + k * pc, 4 * n, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(s.F.data() + k * n, p.F + k * pc, 4 * n, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipMemcpyAsync(s.vgd.data(), p.vgd, 4 * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s.ids.data(), p.ids, 4 * n, hipMemcpyDeviceToHost, st));
+    if (p.dps)
+        for (int k = 0; k < 3; k++)
+            HIP_TRY(hipMemcpyAsync(s.dps.data() + k * n, p.dps + k * pc, 4 * n, hipMemcpyDeviceToHost, st));
+    for (int k = 0; k < 3; k++)
+        HIP_TRY(hipMemcpyAsync(s.fdt.data() + k * n, d->fdt + k * pc, 4 * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (size_t i = 0; i < n; i++)
+        if (s.ids[i] >= n) return fail(SLOSH_E_GRID_OVERFLOW, "particle storage inconsistent (grid overflow?)");
+    return SLOSH_OK;
+}
+
+static StepParams make_params(const slosh_data *d) {
+    StepParams P{};
+    P.gravity[0] = d->params.gravity[0];
+    P.gravity[1] = d->params.gravity[1];
+    P.gravity[2] = d->params.gravity[2];
+    P.dt = d->params.dt;
+    P.h = d->h;
+    P.inv_h = 1.0f / d->h;
+    P.cap = d->cap;
+    P.n = (uint32_t)d->n;
+    P.batched = d->batched;
+    P.num_bodies = d->nbodies;
+    P.has_force = d->has_force;
+    P.has_sand = d->has_sand;
+    P.coll_cache = d->coll_cache;
+    P.cur = (uint32_t)d->cur_g;
+    P.svd_f64 = d->svd_f64;
+    return P;
+}
+
+static ParticleBuffers pbufs(slosh_data *d, int b) {
+    ParticleBuffers r;
+    r.pos = d->pb[b].pos;
+    r.vm = d->pb[b].vm;
+    r.C = d->pb[b].C;
+    r.F = d->pb[b].F;
+    r.vgd = d->pb[b].vgd;
+    r.dps = d->pb[b].dps;
+    r.ids = d->pb[b].ids;
+    return r;
+}
+
+static StaticBuffers sbufs(slosh_data *d) {
+    StaticBuffers s;
+    s.prop = d->s_prop;
+    s.model = d->s_model;
+    s.dp0 = d->s_dp0;
+    s.dp1 = d->s_dp1;
+    return s;
+}
+
+static void record(slosh_timestamps *ts, int pass, hipStream_t st, bool begin) {
+    if (!ts) return;
+    if (begin) {
+        slosh_timestamps::Rec r;
+        r.pass = pass;
+        if (ts->pool.size() >= 2) {
+            r.a = ts->pool.back();
+            ts->pool.pop_back();
+            r.b = ts->pool.back();
+            ts->pool.pop_back();
+        } else {
+            (void)hipEventCreate(&r.a);
+            (void)hipEventCreate(&r.b);
+        }
+        (void)hipEventRecord(r.a, st);
+        ts->recs.push_back(r);
+    } else {
+        (void)hipEventRecord(ts->recs.back().b, st);
+    }
+}
+
+// ---- one substep --------------------------------------------------------------
+static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slosh_hooks *hooks, void *hs,
+                    slosh_timestamps *ts) {
+    const slosh_kernels &K = pl->kernels;
+    const bool any_hook = hooks && (hooks->after_particle_sort || hooks->run_p2g || hooks->run_g2p || hooks->after_p2g ||
+                                    hooks->after_grid_update || hooks->after_g2p || hooks->after_particles_update);
+    if (any_hook && d->nbodies > 0 && !d->coll_cache) {
+        // Hooks may move bodies: from now on keep the reference's per-node cache.
+        for (int i = 0; i < 2; i++)
+            if (!d->coll[i]) TRY(dalloc(&d->coll[i], 2 * (size_t)d->cap * NODES_PER_BLOCK));
+        d->coll_cache = 1;
+        d->coll_prev_valid = 0;
+    }
+    // GpuGrid::swap_buffers (grid.rs:513-517): the table written last substep becomes "previous".
+    const int prev_g = d->cur_g;
+    const int cur_g = prev_g ^ 1;
+    d->cur_g = cur_g;
+    StepParams P = make_params(d);
+    const uint32_t n = (uint32_t)d->n;
+
+    GridBuffers g;
+    g.keys = d->keys[cur_g];
+    g.vals = d->vals[cur_g];
+    g.prev_keys = d->keys[prev_g];
+    g.prev_vals = d->vals[prev_g];
+    g.hdr_key = d->hdr_key;
+    g.nbr = d->nbr;
+    g.cell_cnt = d->cell_cnt;
+    g.cell_start = d->cell_start;
+    g.blk_start = d->blk_start;
+    g.blk_cnt = d->blk_cnt;
+    g.pslot = d->pslot;
+    g.prank = d->prank;
+    g.perm = d->perm;
+    g.counters = d->counters;
+
+    record(ts, PASS_GRID_SORT, st, true);
+    HIP_TRY(launch_grid_build(g, d->pb[d->cur_p].pos, n, d->h, d->batched, d->cap, st));
+    record(ts, PASS_GRID_SORT, st, false);
+    if (hooks && hooks->after_particle_sort && hooks->after_particle_sort(d, st, hs) != 0)
+        return fail(SLOSH_E_HOOK, "after_particle_sort hook failed");
+
+    TransferBuffers t;
+    t.perm = d->perm;
+    t.cell_cnt = d->cell_cnt;
+    t.cell_start = d->cell_start;
+    t.blk_start = d->blk_start;
+    t.blk_cnt = d->blk_cnt;
+    t.hdr_key = d->hdr_key;
+    t.nbr = d->nbr;
+    t.counters = d->counters;
+    t.tiles = d->tiles;
+    t.nodes = d->nodes;
+    t.coll_prev = (d->coll_cache && d->coll_prev_valid) ? (void *)d->coll[prev_g] : nullptr;
+    t.coll_cur = d->coll_cache ? (void *)d->coll[cur_g] : nullptr;
+    t.fdt = d->fdt;
+
+    int handled = 0;
+    if (hooks && hooks->run_p2g && hooks->run_p2g(d, st, hs, &handled) != 0)
+        return fail(SLOSH_E_HOOK, "run_p2g hook failed");
+    const bool p2g_by_hook = handled != 0;
+    if (!p2g_by_hook) {
+        if (!K.builtin_transfers)
+            return fail(SLOSH_E_NO_KERNELS, "pipeline built without builtin transfer kernels; a hook must handle run_p2g");
+        record(ts, PASS_P2G, st, true);
+        HIP_TRY(launch_p2g(P, pbufs(d, d->cur_p), (uint32_t)d->pcap, t, st));
+        record(ts, PASS_P2G, st, false);
+    }
+    if (hooks && hooks->after_p2g && hooks->after_p2g(d, st, hs) != 0) return fail(SLOSH_E_HOOK, "after_p2g hook failed");
+
+    ColliderBuffers cb;
+    cb.shapes = d->shapes;
+    cb.poses = d->poses;
+    cb.mats = d->mats;
+    cb.count = d->nbodies;
+    if (K.grid_update || !p2g_by_hook) {
+        const uint32_t mode = p2g_by_hook ? 1u : 0u;
+        record(ts, PASS_GRID_UPDATE, st, true);
+        HIP_TRY(launch_grid_update(P, t, cb, mode, K.grid_update ? 1u : 0u, st));
+        record(ts, PASS_GRID_UPDATE, st, false);
+    }
+    if (d->coll_cache) d->coll_prev_valid = 1;
+    if (hooks && hooks->after_grid_update && hooks->after_grid_update(d, st, hs) != 0)
+        return fail(SLOSH_E_HOOK, "after_grid_update hook failed");
+
+    handled = 0;
+    if (hooks && hooks->run_g2p && hooks->run_g2p(d, st, hs, &handled) != 0)
+        return fail(SLOSH_E_HOOK, "run_g2p hook failed");
+    const bool g2p_by_hook = handled != 0;
+    const bool split = g2p_by_hook || (hooks && hooks->after_g2p) || !K.particles_update;
+    if (!g2p_by_hook && !K.builtin_transfers)
+        return fail(SLOSH_E_NO_KERNELS, "pipeline built without builtin transfer kernels; a hook must handle run_g2p");
+    if (!split) {
+        // Fused G2P + particle update; writes the next particle buffer in sorted order.
+        record(ts, PASS_G2P, st, true);
+        HIP_TRY(launch_g2p_update(P, pbufs(d, d->cur_p), pbufs(d, d->cur_p ^ 1), sbufs(d), (uint32_t)d->pcap, t, st));
+        record(ts, PASS_G2P, st, false);
+        d->cur_p ^= 1;
+    } else {
+        if (!g2p_by_hook) {
+            record(ts, PASS_G2P, st, true);
+            HIP_TRY(launch_g2p(P, pbufs(d, d->cur_p), (uint32_t)d->pcap, t, st));
+            record(ts, PASS_G2P, st, false);
+        }
+        if (hooks && hooks->after_g2p && hooks->after_g2p(d, st, hs) != 0)
+            return fail(SLOSH_E_HOOK, "after_g2p hook failed");
+        if (K.particles_update) {
+            record(ts, PASS_PARTICLES_UPDATE, st, true);
+            HIP_TRY(launch_particle_update(P, pbufs(d, d->cur_p), sbufs(d), (uint32_t)d->pcap, d->fdt, st));
+            record(ts, PASS_PARTICLES_UPDATE, st, false);
+        }
+    }
+    if (K.particles_update) d->has_force = 0;  // force_dt is zeroed by every particle update
+    if (hooks && hooks->after_particles_update && hooks->after_particles_update(d, st, hs) != 0)
+        return fail(SLOSH_E_HOOK, "after_particles_update hook failed");
+
+    if (d->nbodies > 0) {
+        BodyBuffers bb;
+        bb.poses = d->poses;
+        bb.vels = d->vels;
+        bb.mprops = d->mprops;
+        bb.local_com = d->local_com;
+        bb.impulses = d->impulses;
+        bb.count = d->nbodies;
+        record(ts, PASS_INTEGRATE_BODIES, st, true);
+        HIP_TRY(launch_rigid_update(P, bb, st));
+        record(ts, PASS_INTEGRATE_BODIES, st, false);
+    }
+    return SLOSH_OK;
+}
+
+// ---- C ABI --------------------------------------------------------------------
+extern "C" {
+
+const char *slosh_last_error(void) { return g_last_error.c_str(); }
+const char *slosh_version(void) { return "slosh_hip 0.1 (gfx950)"; }
+
+void slosh_kernels_default(slosh_kernels *out) {
+    if (!out) return;
+    out->builtin_transfers = 1;
+    out->cdf = 1;
+    out->rigid_particles = 1;
+    out->node_reset = 1;
+    out->grid_update = 1;
+    out->particles_update = 1;
+}
+
+int slosh_pipeline_create(const slosh_kernels *kernels, slosh_pipeline **out) {
+    if (!out) return fail(SLOSH_E_INVALID, "null output pointer");
+    slosh_pipeline *p = new slosh_pipeline();
+    if (kernels) p->kernels = *kernels;
+    else slosh_kernels_default(&p->kernels);
+    *out = p;
+    return SLOSH_OK;
+}
+
+void slosh_pipeline_destroy(slosh_pipeline *p) { delete p; }
+
+static void free_data(slosh_data *d) {
+    invalidate_graphs(d);
+    free_particle_buffers(d);
+    for (int i = 0; i < 2; i++) {
+        dfree(d->keys[i]);
+        dfree(d->vals[i]);
+        dfree(d->coll[i]);
+    }
+    dfree(d->hdr_key);
+    dfree(d->nbr);
+    dfree(d->cell_cnt);
+    dfree(d->cell_start);
+    dfree(d->blk_start);
+    dfree(d->blk_cnt);
+    dfree(d->counters);
+    dfree(d->tiles);
+    dfree(d->nodes);
+    dfree(d->shapes);
+    dfree(d->poses);
+    dfree(d->vels);
+    dfree(d->mprops);
+    dfree(d->local_com);
+    dfree(d->mats);
+    dfree(d->impulses);
+    if (d->capture_stream) (void)hipStreamDestroy(d->capture_stream);
+    delete d;
+}
+
+static bool body_may_move(const slosh_body_desc &b) {
+    for (int i = 0; i < 3; i++)
+        if (b.vel.linear[i] != 0.0f || b.vel.angular[i] != 0.0f || b.mprops.inv_mass[i] != 0.0f) return true;
+    return false;
+}
+
+static int upload_bodies(slosh_data *d) {
+    const uint32_t nb = d->nbodies;
+    std::vector<float> shapes(8 * nb), poses(8 * nb), vels(8 * nb), mprops(20 * nb), lcom(4 * nb);
+    std::vector<uint32_t> mats(2 * nb);
+    for (uint32_t i = 0; i < nb; i++) {
+        const slosh_body_desc &b = d->bodies_host[i];
+        std::memcpy(&shapes[8 * i], &b.shape, 32);
+        std::memcpy(&poses[8 * i], &b.pose, 32);
+        for (int c = 0; c < 3; c++) {
+            vels[8 * i + c] = b.vel.linear[c];
+            vels[8 * i + 4 + c] = b.vel.angular[c];
+            lcom[4 * i + c] = b.local_mprops.com[c];
+        }
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 4; c++) mprops[20 * i + 4 * r + c] = b.mprops.inv_inertia[r][c];
+        for (int c = 0; c < 3; c++) {
+            mprops[20 * i + 12 + c] = b.mprops.inv_mass[c];
+            mprops[20 * i + 16 + c] = b.mprops.com[c];
+        }
+    }
+    (void)mats;
+    if (nb == 0) return SLOSH_OK;
+    HIP_TRY(hipMemcpy(d->shapes, shapes.data(), 32 * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d->poses, poses.data(), 32 * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d->vels, vels.data(), 32 * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d->mprops, mprops.data(), 80 * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d->local_com, lcom.data(), 16 * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(d->impulses, 0, sizeof(int) * 6 * nb));
+    return SLOSH_OK;
+}
+
+static int validate_particles(const slosh_particles *p) {
+    if (!p) return fail(SLOSH_E_INVALID, "null particles");
+    if (p->len > 0 && (!p->positions || !p->kinematics || !p->def_grad || !p->properties || !p->models))
+        return fail(SLOSH_E_INVALID, "null particle buffer");
+    if (p->len >= 0xffffffffull / 2) return fail(SLOSH_E_INVALID, "too many particles");
+    return SLOSH_OK;
+}
+
+static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params, const slosh_particles *particles,
+                       size_t particle_capacity, const slosh_body_desc *bodies, size_t num_bodies,
+                       const slosh_boundary_condition *body_materials, float cell_width, uint32_t grid_capacity,
+                       slosh_data *d) {
+    TRY(validate_particles(particles));
+    if (!params || !(cell_width > 0.0f)) return fail(SLOSH_E_INVALID, "invalid params or cell width");
+    if (num_bodies > SLOSH_MAX_COLLIDERS) return fail(SLOSH_E_INVALID, "CPIC only supports up to 16 colliders");
+    if (num_bodies > 0 && (!bodies || !body_materials)) return fail(SLOSH_E_INVALID, "null body buffers");
+    if (grid_capacity == 0 || grid_capacity > (1u << 26)) return fail(SLOSH_E_INVALID, "grid capacity out of range");
+    for (size_t i = 0; i < num_bodies; i++) {
+        uint32_t ty = f2u(bodies[i].shape.a[3]);
+        if (ty == 6u) return fail(SLOSH_E_UNSUPPORTED, "trimesh colliders are not implemented");
+    }
+    d->pipeline = pipeline;
+    d->params = *params;
+    d->h = cell_width;
+    d->cap = next_pow2(grid_capacity);
+    d->n = particles->len;
+    d->pcap = std::max(particle_capacity, particles->len);
+    if (d->pcap == 0) d->pcap = 1;
+    d->batched = particles->scene_ids != nullptr;
+    bool any_force = false, any_sand = false;
+    Staging s;
+    TRY(stage_particles(particles, 0, s, &any_force, &any_sand));
+    d->has_sand = any_sand ? 1u : 0u;
+    d->has_force = any_force ? 1u : 0u;
+    TRY(alloc_particle_buffers(d, d->pcap, d->has_sand != 0));
+    d->models_host.assign(particles->models, particles->models + d->n);
+    d->props_host.assign(particles->properties, particles->properties + d->n);
+    d->scene_host.resize(d->n);
+    for (size_t i = 0; i < d->n; i++) d->scene_host[i] = particles->scene_ids ? particles->scene_ids[i] : 0u;
+    HIP_TRY(hipStreamCreateWithFlags(&d->capture_stream, hipStreamNonBlocking));
+    TRY(upload_staging(d, s, 0, 0, 0, d->capture_stream));
+    const size_t cap = d->cap;
+    for (int i = 0; i < 2; i++) {
+        TRY(dalloc(&d->keys[i], cap));
+        TRY(dalloc(&d->vals[i], cap));
+        HIP_TRY(hipMemset(d->keys[i], 0xff, 4 * cap));
+        HIP_TRY(hipMemset(d->vals[i], 0, 4 * cap));
+    }
+    TRY(dalloc(&d->hdr_key, cap));
+    TRY(dalloc(&d->nbr, cap * NBR_STRIDE));
+    TRY(dalloc(&d->cell_cnt, cap * NODES_PER_BLOCK));
+    TRY(dalloc(&d->cell_start, cap * NODES_PER_BLOCK));
+    TRY(dalloc(&d->blk_start, cap));
+    TRY(dalloc(&d->blk_cnt, cap));
+    TRY(dalloc(&d->counters, 8));
+    HIP_TRY(hipMemset(d->counters, 0, 32));
+    TRY(dalloc(&d->tiles, cap * TILE_NODES));
+    TRY(dalloc(&d->nodes, cap * NODES_PER_BLOCK));
+    d->nbodies = (uint32_t)num_bodies;
+    d->bodies_host.assign(bodies, bodies + num_bodies);
+    bool moving = false;
+    for (size_t i = 0; i < num_bodies; i++) moving = moving || body_may_move(bodies[i]);
+    d->coll_cache = moving ? 1u : 0u;
+    d->coll_prev_valid = 1;  // the first substep's previous table is empty (grid.rs:454-457)
+    if (d->coll_cache)
+        for (int i = 0; i < 2; i++) TRY(dalloc(&d->coll[i], 2 * cap * NODES_PER_BLOCK));
+    TRY(dalloc(&d->shapes, 2 * std::max<size_t>(num_bodies, 1)));
+    TRY(dalloc(&d->poses, 2 * std::max<size_t>(num_bodies, 1)));
+    TRY(dalloc(&d->vels, 2 * std::max<size_t>(num_bodies, 1)));
+    TRY(dalloc(&d->mprops, 5 * std::max<size_t>(num_bodies, 1)));
+    TRY(dalloc(&d->local_com, std::max<size_t>(num_bodies, 1)));
+    TRY(dalloc(&d->mats, std::max<size_t>(num_bodies, 1)));
+    TRY(dalloc(&d->impulses, 6 * std::max<size_t>(num_bodies, 1)));
+    if (num_bodies) {
+        std::vector<uint2> m(num_bodies);
+        for (size_t i = 0; i < num_bodies; i++) m[i] = make_uint2(body_materials[i].ty, f2u(body_materials[i].friction));
+        HIP_TRY(hipMemcpy(d->mats, m.data(), 8 * num_bodies, hipMemcpyHostToDevice));
+    }
+    TRY(upload_bodies(d));
+    HIP_TRY(hipDeviceSynchronize());
+    return SLOSH_OK;
+}
+
+int slosh_data_create(slosh_pipeline *pipeline, const slosh_sim_params *params, const slosh_particles *particles,
+                      size_t particle_capacity, const slosh_body_desc *bodies, size_t num_bodies,
+                      const slosh_boundary_condition *body_materials, float cell_width, uint32_t grid_capacity,
+                      slosh_data **out) {
+    if (!pipeline || !out) return fail(SLOSH_E_INVALID, "null pipeline or output");
+    *out = nullptr;
+    slosh_data *d = new slosh_data();
+    int rc = create_impl(pipeline, params, particles, particle_capacity, bodies, num_bodies, body_materials, cell_width,
+                         grid_capacity, d);
+    if (rc != SLOSH_OK) {
+        std::string msg = g_last_error;
+        free_data(d);
+        g_last_error = msg;
+        return rc;
+    }
+    *out = d;
+    return SLOSH_OK;
+}
+
+void slosh_data_destroy(slosh_data *d) {
+    if (d) free_data(d);
+}
+
+int slosh_launch_step(slosh_pipeline *pipeline, slosh_data *d, slosh_stream stream, const slosh_hooks *hooks,
+                      void *hooks_state, slosh_timestamps *ts) {
+    if (!pipeline || !d) return fail(SLOSH_E_INVALID, "null pipeline or data");
+    return run_step(pipeline, d, (hipStream_t)stream, hooks, hooks_state, ts);
+}
+
+int slosh_launch_steps_graph(slosh_pipeline *pipeline, slosh_data *d, slosh_stream stream, uint32_t count) {
+    if (!pipeline || !d) return fail(SLOSH_E_INVALID, "null pipeline or data");
+    hipStream_t st = (hipStream_t)stream;
+    while (count >= 2) {
+        // The substep alternates particle buffers and hash tables, so a pair of
+        // substeps returns to the same parity; one graph per starting parity.
+        const int slot = (d->cur_p & 1) | ((d->cur_g & 1) << 1);
+        if (d->has_force || (d->coll_cache && !d->coll_prev_valid)) {
+            TRY(run_step(pipeline, d, st, nullptr, nullptr, nullptr));
+            count--;
+            continue;
+        }
+        if (!d->graph[slot]) {
+            const int save_p = d->cur_p, save_g = d->cur_g;
+            const uint32_t save_force = d->has_force;
+            HIP_TRY(hipStreamBeginCapture(d->capture_stream, hipStreamCaptureModeThreadLocal));
+            int rc = run_step(pipeline, d, d->capture_stream, nullptr, nullptr, nullptr);
+            if (rc == SLOSH_OK) rc = run_step(pipeline, d, d->capture_stream, nullptr, nullptr, nullptr);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(d->capture_stream, &graph);
+            d->cur_p = save_p;
+            d->cur_g = save_g;
+            d->has_force = save_force;
+            if (rc != SLOSH_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(SLOSH_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&d->graph[slot], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(SLOSH_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        HIP_TRY(hipGraphLaunch(d->graph[slot], st));
+        count -= 2;
+    }
+    if (count == 1) TRY(run_step(pipeline, d, st, nullptr, nullptr, nullptr));
+    return SLOSH_OK;
+}
+
+int slosh_set_svd_fp64(slosh_data *d, int enable) {
+    if (!d) return fail(SLOSH_E_INVALID, "null data");
+    d->svd_f64 = enable ? 1u : 0u;
+    invalidate_graphs(d);
+    return SLOSH_OK;
+}
+
+int slosh_set_sim_params(slosh_data *d, const slosh_sim_params *params) {
+    if (!d || !params) return fail(SLOSH_E_INVALID, "null argument");
+    d->params = *params;
+    invalidate_graphs(d);
+    return SLOSH_OK;
+}
+
+int slosh_check_errors(slosh_data *d, slosh_stream stream) {
+    if (!d) return fail(SLOSH_E_INVALID, "null data");
+    hipStream_t st = (hipStream_t)stream;
+    uint32_t c[8];
+    HIP_TRY(hipMemcpyAsync(c, d->counters, 32, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (c[CTR_ERRORS] != 0u) {
+        uint32_t zero = 0;
+        HIP_TRY(hipMemcpy(d->counters + CTR_ERRORS, &zero, 4, hipMemcpyHostToDevice));
+        return fail(SLOSH_E_GRID_OVERFLOW, "sparse-grid hash map overflow (increase grid_capacity)");
+    }
+    return SLOSH_OK;
+}
+
+size_t slosh_num_particles(const slosh_data *d) { return d ? d->n : 0; }
+
+int slosh_read_positions(slosh_data *d, slosh_stream stream, float *out4) {
+    if (!d || (!out4 && d->n)) return fail(SLOSH_E_INVALID, "null argument");
+    Snapshot s;
+    TRY(download_current(d, (hipStream_t)stream, s));
+    for (size_t i = 0; i < d->n; i++) {
+        uint32_t id = s.ids[i];
+        out4[4 * id + 0] = s.pos[4 * i + 0];
+        out4[4 * id + 1] = s.pos[4 * i + 1];
+        out4[4 * id + 2] = s.pos[4 * i + 2];
+        out4[4 * id + 3] = 0.0f;
+    }
+    return SLOSH_OK;
+}
+
+int slosh_read_kinematics(slosh_data *d, slosh_stream stream, slosh_kinematics *out) {
+    if (!d || (!out && d->n)) return fail(SLOSH_E_INVALID, "null argument");
+    Snapshot s;
+    TRY(download_current(d, (hipStream_t)stream, s));
+    const size_t n = d->n;
+    for (size_t i = 0; i < n; i++) {
+        slosh_kinematics &k = out[s.ids[i]];
+        std::memset(&k, 0, sizeof k);
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) k.affine.m[r][c] = s.C[(size_t)(r * 3 + c) * n + i];
+        for (int c = 0; c < 3; c++) {
+            k.velocity[c] = s.vm[4 * i + c];
+            k.force_dt[c] = s.fdt[(size_t)c * n + i];
+        }
+        k.vel_grad_det = s.vgd[i];
+        k.mass = s.vm[4 * i + 3];
+        k.enabled = f2u(s.pos[4 * i + 3]) & 1u;
+    }
+    return SLOSH_OK;
+}
+
+int slosh_read_def_grad(slosh_data *d, slosh_stream stream, slosh_matrix *out) {
+    if (!d || (!out && d->n)) return fail(SLOSH_E_INVALID, "null argument");
+    Snapshot s;
+    TRY(download_current(d, (hipStream_t)stream, s));
+    const size_t n = d->n;
+    for (size_t i = 0; i < n; i++) {
+        slosh_matrix &m = out[s.ids[i]];
+        std::memset(&m, 0, sizeof m);
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) m.m[r][c] = s.F[(size_t)(r * 3 + c) * n + i];
+    }
+    return SLOSH_OK;
+}
+
+int slosh_read_models(slosh_data *d, slosh_stream stream, slosh_particle_model *out) {
+    if (!d || (!out && d->n)) return fail(SLOSH_E_INVALID, "null argument");
+    Snapshot s;
+    TRY(download_current(d, (hipStream_t)stream, s));
+    const size_t n = d->n;
+    for (size_t i = 0; i < n; i++) {
+        uint32_t id = s.ids[i];
+        out[id] = d->models_host[id];
+        if (model_is_sand(out[id]))
+            for (int c = 0; c < 3; c++) out[id].words[1 + c] = f2u(s.dps[(size_t)c * n + i]);
+    }
+    return SLOSH_OK;
+}
+
+int slosh_read_properties(slosh_data *d, slosh_stream stream, slosh_particle_properties *out) {
+    (void)stream;
+    if (!d || (!out && d->n)) return fail(SLOSH_E_INVALID, "null argument");
+    std::memcpy(out, d->props_host.data(), sizeof(slosh_particle_properties) * d->n);
+    return SLOSH_OK;
+}
+
+int slosh_read_scene_ids(slosh_data *d, slosh_stream stream, uint32_t *out) {
+    (void)stream;
+    if (!d || (!out && d->n)) return fail(SLOSH_E_INVALID, "null argument");
+    std::memcpy(out, d->scene_host.data(), 4 * d->n);
+    return SLOSH_OK;
+}
+
+int slosh_read_bodies(slosh_data *d, slosh_stream stream, slosh_sim3 *poses, slosh_velocity *vels) {
+    if (!d) return fail(SLOSH_E_INVALID, "null data");
+    const uint32_t nb = d->nbodies;
+    if (nb == 0) return SLOSH_OK;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<float> p(8 * nb), v(8 * nb);
+    HIP_TRY(hipMemcpyAsync(p.data(), d->poses, 32 * nb, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(v.data(), d->vels, 32 * nb, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < nb; i++) {
+        if (poses) std::memcpy(&poses[i], &p[8 * i], 32);
+        if (vels) {
+            std::memset(&vels[i], 0, sizeof(slosh_velocity));
+            for (int c = 0; c < 3; c++) {
+                vels[i].linear[c] = v[8 * i + c];
+                vels[i].angular[c] = v[8 * i + 4 + c];
+            }
+        }
+    }
+    return SLOSH_OK;
+}
+
+int slosh_num_blocks(slosh_data *d, slosh_stream stream, uint32_t *out) {
+    if (!d || !out) return fail(SLOSH_E_INVALID, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    uint32_t c[8];
+    HIP_TRY(hipMemcpyAsync(c, d->counters, 32, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *out = c[CTR_NUM_BLOCKS];
+    return SLOSH_OK;
+}
+
+static const int ASSOC_H[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1}, {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+
+int slosh_read_blocks(slosh_data *d, slosh_stream stream, slosh_block_info *out, uint32_t max_blocks,
+                      uint32_t *num_written) {
+    if (!d || !out) return fail(SLOSH_E_INVALID, "null argument");
+    uint32_t B = 0;
+    TRY(slosh_num_blocks(d, stream, &B));
+    hipStream_t st = (hipStream_t)stream;
+    B = std::min(B, max_blocks);
+    std::vector<uint32_t> keys(B), cnt((size_t)B * 64), nbr((size_t)B * NBR_STRIDE), start(B);
+    if (B) {
+        HIP_TRY(hipMemcpyAsync(keys.data(), d->hdr_key, 4 * B, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(cnt.data(), d->cell_cnt, 4 * (size_t)B * 64, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(nbr.data(), d->nbr, 4 * (size_t)B * NBR_STRIDE, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(start.data(), d->blk_start, 4 * B, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    for (uint32_t b = 0; b < B; b++) {
+        slosh_block_info &bi = out[b];
+        std::memset(&bi, 0, sizeof bi);
+        uint32_t scene;
+        unpack_key(d->batched, keys[b], &bi.vid[0], &bi.vid[1], &bi.vid[2], &scene);
+        bi.first_particle = start[b];
+        // Primary buckets: the associated-cell slab along z (sort.slang:15-21).
+        for (int c = 0; c < 64; c++) bi.bucket_counts[c >> 4] += cnt[(size_t)b * 64 + c];
+        // Extras received from the -1 neighbour B - o: its particles whose
+        // associated cell is >= 2 along every axis where o = 1 (sort.slang:27-42).
+        for (int o = 1; o < 8; o++) {
+            uint32_t src = nbr[(size_t)b * NBR_STRIDE + NBR_MINUS + o - 1];
+            if (src == SLOSH_NONE || src >= B) continue;
+            for (int c = 0; c < 64; c++) {
+                int lx = c & 3, ly = (c >> 2) & 3, lz = c >> 4;
+                if ((ASSOC_H[o][0] && lx < 2) || (ASSOC_H[o][1] && ly < 2) || (ASSOC_H[o][2] && lz < 2)) continue;
+                int local = lz - ASSOC_H[o][2] * 4;
+                int bucket = 4 + std::max(local, -2) + 2;
+                bi.bucket_counts[bucket] += cnt[(size_t)src * 64 + c];
+            }
+        }
+        for (int k = 0; k < 4; k++) bi.num_particles += bi.bucket_counts[k];
+        bi.num_particles_with_extras = bi.num_particles;
+        for (int k = 4; k < 10; k++) bi.num_particles_with_extras += bi.bucket_counts[k];
+    }
+    if (num_written) *num_written = B;
+    return SLOSH_OK;
+}
+
+int slosh_read_nodes(slosh_data *d, slosh_stream stream, float *out4, uint32_t max_blocks) {
+    if (!d || !out4) return fail(SLOSH_E_INVALID, "null argument");
+    uint32_t B = 0;
+    TRY(slosh_num_blocks(d, stream, &B));
+    B = std::min(B, max_blocks);
+    if (B) {
+        HIP_TRY(hipMemcpyAsync(out4, d->nodes, 16 * (size_t)B * 64, hipMemcpyDeviceToHost, (hipStream_t)stream));
+        HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    }
+    return SLOSH_OK;
+}
+
+int slosh_read_block_scenes(slosh_data *d, slosh_stream stream, uint32_t *out, uint32_t max_blocks) {
+    if (!d || !out) return fail(SLOSH_E_INVALID, "null argument");
+    uint32_t B = 0;
+    TRY(slosh_num_blocks(d, stream, &B));
+    B = std::min(B, max_blocks);
+    std::vector<uint32_t> keys(B);
+    if (B) {
+        HIP_TRY(hipMemcpyAsync(keys.data(), d->hdr_key, 4 * B, hipMemcpyDeviceToHost, (hipStream_t)stream));
+        HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    }
+    for (uint32_t b = 0; b < B; b++) {
+        int x, y, z;
+        unpack_key(d->batched, keys[b], &x, &y, &z, &out[b]);
+    }
+    return SLOSH_OK;
+}
+
+// Rebuilds every particle buffer from host arrays in original order (used by
+// append beyond capacity and by removals).
+static int rebuild(slosh_data *d, hipStream_t st, size_t new_cap, const Staging &all, bool any_sand) {
+    invalidate_graphs(d);
+    HIP_TRY(hipStreamSynchronize(st));
+    free_particle_buffers(d);
+    d->pcap = std::max<size_t>(new_cap, std::max<size_t>(all.n, 1));
+    d->n = all.n;
+    d->has_sand = (d->has_sand || any_sand) ? 1u : 0u;
+    TRY(alloc_particle_buffers(d, d->pcap, d->has_sand != 0));
+    d->cur_p = 0;
+    return upload_staging(d, all, 0, 0, 0, st);
+}
+
+// Current state as a Staging in original order (static parts from host copies).
+static int gather_all(slosh_data *d, hipStream_t st, Staging &all, bool *any_force) {
+    Snapshot s;
+    TRY(download_current(d, st, s));
+    const size_t n = d->n;
+    std::vector<slosh_kinematics> kin(n);
+    std::vector<slosh_matrix> F(n);
+    std::vector<float> pos(4 * n);
+    std::vector<slosh_particle_model> models(n);
+    for (size_t i = 0; i < n; i++) {
+        uint32_t id = s.ids[i];
+        for (int c = 0; c < 3; c++) pos[4 * id + c] = s.pos[4 * i + c];
+        slosh_kinematics &k = kin[id];
+        std::memset(&k, 0, sizeof k);
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                k.affine.m[r][c] = s.C[(size_t)(r * 3 + c) * n + i];
+                F[id].m[r][c] = s.F[(size_t)(r * 3 + c) * n + i];
+            }
+        for (int c = 0; c < 3; c++) {
+            k.velocity[c] = s.vm[4 * i + c];
+            k.force_dt[c] = s.fdt[(size_t)c * n + i];
+        }
+        k.vel_grad_det = s.vgd[i];
+        k.mass = s.vm[4 * i + 3];
+        k.enabled = f2u(s.pos[4 * i + 3]) & 1u;
+        models[id] = d->models_host[id];
+        if (model_is_sand(models[id]))
+            for (int c = 0; c < 3; c++) models[id].words[1 + c] = f2u(s.dps[(size_t)c * n + i]);
+    }
+    d->models_host = models;
+    slosh_particles p;
+    p.len = n;
+    p.positions = pos.data();
+    p.kinematics = kin.data();
+    p.def_grad = F.data();
+    p.properties = d->props_host.data();
+    p.models = models.data();
+    p.scene_ids = d->batched ? d->scene_host.data() : nullptr;
+    bool any_sand = false;
+    return stage_particles(&p, 0, all, any_force, &any_sand);
+}
+
+static void concat_staging(Staging &a, const Staging &b) {
+    Staging r;
+    r.resize(a.n + b.n);
+    auto cat4 = [&](std::vector<float> &dst, const std::vector<float> &x, const std::vector<float> &y) {
+        std::copy(x.begin(), x.end(), dst.begin());
+        std::copy(y.begin(), y.end(), dst.begin() + x.size());
+    };
+    cat4(r.pos, a.pos, b.pos);
+    cat4(r.vm, a.vm, b.vm);
+    cat4(r.prop, a.prop, b.prop);
+    cat4(r.model, a.model, b.model);
+    cat4(r.dp0, a.dp0, b.dp0);
+    cat4(r.dp1, a.dp1, b.dp1);
+    cat4(r.vgd, a.vgd, b.vgd);
+    auto catk = [&](std::vector<float> &dst, const std::vector<float> &x, const std::vector<float> &y, int comps) {
+        for (int k = 0; k < comps; k++) {
+            std::copy(x.begin() + k * a.n, x.begin() + (k + 1) * a.n, dst.begin() + k * r.n);
+            std::copy(y.begin() + k * b.n, y.begin() + (k + 1) * b.n, dst.begin() + k * r.n + a.n);
+        }
+    };
+    catk(r.C, a.C, b.C, 9);
+    catk(r.F, a.F, b.F, 9);
+    catk(r.dps, a.dps, b.dps, 3);
+    catk(r.fdt, a.fdt, b.fdt, 3);
+    std::copy(a.ids.begin(), a.ids.end(), r.ids.begin());
+    for (size_t i = 0; i < b.n; i++) r.ids[a.n + i] = (uint32_t)(a.n + i);
+    a = std::move(r);
+}
+
+int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_particles *particles) {
+    if (!d) return fail(SLOSH_E_INVALID, "null data");
+    TRY(validate_particles(particles));
+    if ((particles->scene_ids != nullptr) != (d->batched != 0))
+        return fail(SLOSH_E_INVALID, "scene ids must be given iff the data is batched");
+    hipStream_t st = (hipStream_t)stream;
+    if (particles->len == 0) return SLOSH_OK;
+    bool any_force = false, any_sand = false;
+    Staging s;
+    TRY(stage_particles(particles, (uint32_t)d->n, s, &any_force, &any_sand));
+    invalidate_graphs(d);
+    d->models_host.insert(d->models_host.end(), particles->models, particles->models + particles->len);
+    d->props_host.insert(d->props_host.end(), particles->properties, particles->properties + particles->len);
+    for (size_t i = 0; i < particles->len; i++)
+        d->scene_host.push_back(particles->scene_ids ? particles->scene_ids[i] : 0u);
+    const bool need_sand = any_sand && !d->has_sand;
+    if (d->n + particles->len <= d->pcap && !need_sand) {
+        TRY(upload_staging(d, s, d->cur_p, d->n, d->n, st));
+        d->n += particles->len;
+    } else {
+        // GpuTensor::append reallocates; we rebuild in original order with headroom.
+        Staging all;
+        bool f2 = false;
+        // models_host already holds the appended records: gather the old part only.
+        const size_t old_n = d->n;
+        std::vector<slosh_particle_model> saved(d->models_host.begin() + old_n, d->models_host.end());
+        d->models_host.resize(old_n);
+        TRY(gather_all(d, st, all, &f2));
+        d->models_host.insert(d->models_host.end(), saved.begin(), saved.end());
+        concat_staging(all, s);
+        any_force = any_force || f2;
+        TRY(rebuild(d, st, (d->n + particles->len) * 3 / 2 + 1024, all, any_sand));
+    }
+    if (any_force) d->has_force = 1;
+    return SLOSH_OK;
+}
+
+static int remove_ids(slosh_data *d, hipStream_t st, const std::vector<uint8_t> &keep) {
+    Staging all;
+    bool any_force = false;
+    TRY(gather_all(d, st, all, &any_force));
+    const size_t n = all.n;
+    size_t m = 0;
+    for (size_t i = 0; i < n; i++) m += keep[i];
+    Staging r;
+    r.resize(m);
+    std::vector<slosh_particle_model> models;
+    std::vector<slosh_particle_properties> props;
+    std::vector<uint32_t> scenes;
+    size_t j = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (!keep[i]) continue;
+        for (int c = 0; c < 4; c++) {
+            r.pos[4 * j + c] = all.pos[4 * i + c];
+            r.vm[4 * j + c] = all.vm[4 * i + c];
+            r.prop[4 * j + c] = all.prop[4 * i + c];
+            r.model[4 * j + c] = all.model[4 * i + c];
+            r.dp0[4 * j + c] = all.dp0[4 * i + c];
+            r.dp1[4 * j + c] = all.dp1[4 * i + c];
+        }
+        for (int k = 0; k < 9; k++) {
+            r.C[k * m + j] = all.C[k * n + i];
+            r.F[k * m + j] = all.F[k * n + i];
+        }
+        for (int k = 0; k < 3; k++) {
+            r.dps[k * m + j] = all.dps[k * n + i];
+            r.fdt[k * m + j] = all.fdt[k * n + i];
+        }
+        r.vgd[j] = all.vgd[i];
+        r.ids[j] = (uint32_t)j;
+        models.push_back(d->models_host[i]);
+        props.push_back(d->props_host[i]);
+        scenes.push_back(d->scene_host[i]);
+        j++;
+    }
+    d->models_host = models;
+    d->props_host = props;
+    d->scene_host = scenes;
+    TRY(rebuild(d, st, d->pcap, r, false));
+    d->has_force = any_force ? 1u : 0u;
+    return SLOSH_OK;
+}
+
+int slosh_remove_particles(slosh_data *d, slosh_stream stream, size_t start, size_t end) {
+    if (!d) return fail(SLOSH_E_INVALID, "null data");
+    if (start > end || end > d->n) return fail(SLOSH_E_INVALID, "removal range out of bounds");
+    if (start == end) return SLOSH_OK;
+    std::vector<uint8_t> keep(d->n, 1);
+    for (size_t i = start; i < end; i++) keep[i] = 0;
+    return remove_ids(d, (hipStream_t)stream, keep);
+}
+
+int slosh_remove_scene(slosh_data *d, slosh_stream stream, uint32_t scene_id, size_t *removed) {
+    if (!d) return fail(SLOSH_E_INVALID, "null data");
+    std::vector<uint8_t> keep(d->n, 1);
+    size_t cnt = 0;
+    for (size_t i = 0; i < d->n; i++)
+        if (d->scene_host[i] == scene_id) {
+            keep[i] = 0;
+            cnt++;
+        }
+    if (removed) *removed = cnt;
+    if (cnt == 0) return SLOSH_OK;
+    return remove_ids(d, (hipStream_t)stream, keep);
+}
+
+int slosh_write_kinematics(slosh_data *d, slosh_stream stream, const slosh_kinematics *kin) {
+    if (!d || (!kin && d->n)) return fail(SLOSH_E_INVALID, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    Snapshot s;
+    TRY(download_current(d, st, s));
+    const size_t n = d->n, pc = d->pcap;
+    bool any_force = false;
+    for (size_t i = 0; i < n; i++) {
+        const slosh_kinematics &k = kin[s.ids[i]];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) s.C[(size_t)(r * 3 + c) * n + i] = k.affine.m[r][c];
+        for (int c = 0; c < 3; c++) {
+            s.vm[4 * i + c] = k.velocity[c];
+            s.fdt[(size_t)c * n + i] = k.force_dt[c];
+            if (k.force_dt[c] != 0.0f) any_force = true;
+        }
+        s.vm[4 * i + 3] = k.mass;
+        s.vgd[i] = k.vel_grad_det;
+        uint32_t w = f2u(s.pos[4 * i + 3]);
+        s.pos[4 * i + 3] = u2f((w & ~1u) | (k.enabled ? 1u : 0u));
+    }
+    PBufs &p = d->pb[d->cur_p];
+    if (n) {
+        HIP_TRY(hipMemcpyAsync(p.pos, s.pos.data(), 16 * n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(p.vm, s.vm.data(), 16 * n, hipMemcpyHostToDevice, st));
+        for (int k = 0; k < 9; k++) HIP_TRY(hipMemcpyAsync(p.C + k * pc, s.C.data() + k * n, 4 * n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(p.vgd, s.vgd.data(), 4 * n, hipMemcpyHostToDevice, st));
+        for (int k = 0; k < 3; k++)
+            HIP_TRY(hipMemcpyAsync(d->fdt + k * pc, s.fdt.data() + k * n, 4 * n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    if (any_force) {
+        d->has_force = 1;
+        invalidate_graphs(d);
+    }
+    return SLOSH_OK;
+}
+
+int slosh_device_view_get(slosh_data *d, slosh_device_view *out) {
+    if (!d || !out) return fail(SLOSH_E_INVALID, "null argument");
+    PBufs &p = d->pb[d->cur_p];
+    out->num_particles = d->n;
+    out->particle_capacity = d->pcap;
+    out->grid_capacity = d->cap;
+    out->cell_width = d->h;
+    out->positions = p.pos;
+    out->vel_mass = p.vm;
+    out->affine = p.C;
+    out->def_grad = p.F;
+    out->vel_grad_det = p.vgd;
+    out->particle_ids = p.ids;
+    out->force_dt = d->fdt;
+    out->nodes = d->nodes;
+    out->sorted_ids = d->perm;
+    out->counters = d->counters;
+    return SLOSH_OK;
+}
+
+// ---- timestamps / streams / events -------------------------------------------
+int slosh_timestamps_create(slosh_timestamps **out) {
+    if (!out) return fail(SLOSH_E_INVALID, "null output");
+    *out = new slosh_timestamps();
+    return SLOSH_OK;
+}
+
+void slosh_timestamps_destroy(slosh_timestamps *ts) {
+    if (!ts) return;
+    for (auto &r : ts->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : ts->pool) (void)hipEventDestroy(e);
+    delete ts;
+}
+
+int slosh_timestamps_read(slosh_timestamps *ts, double *ms, uint32_t *count) {
+    if (!ts) return fail(SLOSH_E_INVALID, "null timestamps");
+    for (auto &r : ts->recs) {
+        HIP_TRY(hipEventSynchronize(r.b));
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
+        ts->acc[r.pass] += t;
+        ts->count[r.pass] += 1;
+        ts->pool.push_back(r.a);
+        ts->pool.push_back(r.b);
+    }
+    ts->recs.clear();
+    for (int i = 0; i < SLOSH_NUM_PASSES; i++) {
+        if (ms) ms[i] = ts->acc[i];
+        if (count) count[i] = ts->count[i];
+    }
+    return SLOSH_OK;
+}
+
+void slosh_timestamps_reset(slosh_timestamps *ts) {
+    if (!ts) return;
+    for (int i = 0; i < SLOSH_NUM_PASSES; i++) {
+        ts->acc[i] = 0.0;
+        ts->count[i] = 0;
+    }
+}
+
+int slosh_stream_create(slosh_stream *out) {
+    if (!out) return fail(SLOSH_E_INVALID, "null output");
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *out = (slosh_stream)s;
+    return SLOSH_OK;
+}
+
+int slosh_stream_destroy(slosh_stream stream) {
+    if (stream) HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return SLOSH_OK;
+}
+
+int slosh_stream_synchronize(slosh_stream stream) {
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return SLOSH_OK;
+}
+
+int slosh_event_create(void **out) {
+    if (!out) return fail(SLOSH_E_INVALID, "null output");
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    *out = (void *)e;
+    return SLOSH_OK;
+}
+
+int slosh_event_destroy(void *event) {
+    if (event) HIP_TRY(hipEventDestroy((hipEvent_t)event));
+    return SLOSH_OK;
+}
+
+int slosh_event_record(void *event, slosh_stream stream) {
+    HIP_TRY(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return SLOSH_OK;
+}
+
+int slosh_event_elapsed_ms(void *start, void *stop, float *ms) {
+    HIP_TRY(hipEventSynchronize((hipEvent_t)stop));
+    HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return SLOSH_OK;
+}
+
+int slosh_device_count(int *out) {
+    if (!out) return fail(SLOSH_E_INVALID, "null output");
+    HIP_TRY(hipGetDeviceCount(out));
+    return SLOSH_OK;
+}
+
+}  // extern "C"

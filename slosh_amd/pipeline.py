@@ -1,0 +1,334 @@
+"""Python mirror of slosh's pipeline API over the C ABI (include/slosh_hip.h).
+
+* ``MpmPipeline``  -- ``MpmPipeline::new`` / ``new_with_kernels`` + ``launch_step``
+                      (src/pipeline.rs:456-770)
+* ``MpmPipelineKernels`` -- kernel-family flags (src/pipeline.rs:39-93)
+* ``MpmData``      -- ``MpmData::new`` (src/pipeline.rs:311-353) plus the readback,
+                      append and removal entry points of ``GpuParticles``
+                      (src/solver/particle.rs:505-602)
+* ``MpmPipelineHooks`` -- the six callbacks of src/pipeline.rs:146-238
+
+Every call goes through libslosh_hip.so; there is no host fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from . import layouts as L
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Stream:
+    """A HIP stream owned by the library (non-blocking, default priority)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _lib.check(_lib.load().slosh_stream_create(C.byref(h)))
+        self.handle = h
+
+    def synchronize(self):
+        _lib.check(_lib.load().slosh_stream_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            _lib.load().slosh_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        h = C.c_void_p()
+        _lib.check(_lib.load().slosh_event_create(C.byref(h)))
+        self.handle = h
+
+    def record(self, stream):
+        _lib.check(_lib.load().slosh_event_record(self.handle, stream.handle if stream else None))
+
+    def elapsed_ms(self, stop):
+        ms = C.c_float()
+        _lib.check(_lib.load().slosh_event_elapsed_ms(self.handle, stop.handle, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            _lib.load().slosh_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class MpmPipelineKernels:
+    def __init__(self, builtin_transfers=True, cdf=True, rigid_particles=True, node_reset=True, grid_update=True,
+                 particles_update=True):
+        self.builtin_transfers = builtin_transfers
+        self.cdf = cdf
+        self.rigid_particles = rigid_particles
+        self.node_reset = node_reset
+        self.grid_update = grid_update
+        self.particles_update = particles_update
+
+    @classmethod
+    def hooks_only(cls):
+        return cls(builtin_transfers=False, cdf=False, rigid_particles=False)
+
+    def to_c(self):
+        return _lib.Kernels(*(int(bool(getattr(self, f))) for f, _ in _lib.Kernels._fields_))
+
+
+class MpmPipelineHooks:
+    """Override any of these; return None/0 for success. ``run_p2g`` / ``run_g2p``
+    return True when they handled the pass (the built-in pass is then skipped)."""
+
+    def after_particle_sort(self, data, stream): pass
+    def run_p2g(self, data, stream): return False
+    def run_g2p(self, data, stream): return False
+    def after_p2g(self, data, stream): pass
+    def after_grid_update(self, data, stream): pass
+    def after_g2p(self, data, stream): pass
+    def after_particles_update(self, data, stream): pass
+
+    def _to_c(self, data):
+        def wrap(name):
+            fn = getattr(self, name)
+            if getattr(type(self), name) is getattr(MpmPipelineHooks, name):
+                return _lib.HOOK_FN()
+            def cb(_d, s, _state):
+                try:
+                    fn(data, s)
+                    return 0
+                except Exception:
+                    return -5
+            return _lib.HOOK_FN(cb)
+
+        def wrap_replace(name):
+            fn = getattr(self, name)
+            if getattr(type(self), name) is getattr(MpmPipelineHooks, name):
+                return _lib.REPLACE_FN()
+            def cb(_d, s, _state, handled):
+                try:
+                    handled[0] = 1 if fn(data, s) else 0
+                    return 0
+                except Exception:
+                    return -5
+            return _lib.REPLACE_FN(cb)
+
+        h = _lib.Hooks(wrap("after_particle_sort"), wrap_replace("run_p2g"), wrap_replace("run_g2p"),
+                       wrap("after_p2g"), wrap("after_grid_update"), wrap("after_g2p"),
+                       wrap("after_particles_update"))
+        return h
+
+
+class MpmPipeline:
+    def __init__(self, kernels=None):
+        kernels = kernels or MpmPipelineKernels()
+        self._k = kernels.to_c()
+        h = C.c_void_p()
+        _lib.check(_lib.load().slosh_pipeline_create(C.byref(self._k), C.byref(h)))
+        self.handle = h
+
+    def launch_step(self, data, stream=None, hooks=None, timestamps=None):
+        """Enqueues one substep (MpmPipeline::launch_step); never synchronizes."""
+        ch = hooks._to_c(data) if hooks is not None else None
+        data._hooks_keepalive = ch
+        _lib.check(_lib.load().slosh_launch_step(self.handle, data.handle, stream.handle if stream else None,
+                                                 C.byref(ch) if ch is not None else None, None,
+                                                 timestamps.handle if timestamps else None))
+
+    def launch_steps_graph(self, data, count, stream=None):
+        """Enqueues ``count`` substeps by replaying a captured HIP graph."""
+        _lib.check(_lib.load().slosh_launch_steps_graph(self.handle, data.handle, stream.handle if stream else None,
+                                                        int(count)))
+
+    def __del__(self):
+        try:
+            _lib.load().slosh_pipeline_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Timestamps:
+    PASSES = ["grid_sort", "p2g", "grid_update", "g2p", "particles_update", "integrate_bodies"]
+
+    def __init__(self):
+        h = C.c_void_p()
+        _lib.check(_lib.load().slosh_timestamps_create(C.byref(h)))
+        self.handle = h
+
+    def read(self):
+        ms = np.zeros(6, np.float64)
+        cnt = np.zeros(6, np.uint32)
+        _lib.check(_lib.load().slosh_timestamps_read(self.handle, _p(ms), _p(cnt)))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.PASSES)}
+
+    def reset(self):
+        _lib.load().slosh_timestamps_reset(self.handle)
+
+    def __del__(self):
+        try:
+            _lib.load().slosh_timestamps_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def _particles_struct(particles, scene_ids=None):
+    keep = {k: np.ascontiguousarray(particles[k]) for k in ("pos4", "kin", "def_grad", "props", "models")}
+    assert keep["kin"].dtype == L.KINEMATICS and keep["props"].dtype == L.PROPERTIES
+    keep["models"] = keep["models"].astype(np.uint32, copy=False)
+    if scene_ids is not None:
+        keep["scene"] = np.ascontiguousarray(scene_ids, dtype=np.uint32)
+    n = keep["pos4"].shape[0]
+    st = _lib.Particles(n, keep["pos4"].ctypes.data, keep["kin"].ctypes.data, keep["def_grad"].ctypes.data,
+                        keep["props"].ctypes.data, keep["models"].ctypes.data,
+                        keep["scene"].ctypes.data if scene_ids is not None else None)
+    return st, keep
+
+
+class MpmData:
+    """Device-resident simulation state (MpmData::new)."""
+
+    def __init__(self, pipeline, params, particles, bodies, materials, cell_width, grid_capacity,
+                 scene_ids=None, particle_capacity=0):
+        st, keep = _particles_struct(particles, scene_ids)
+        self._params = np.array([params], dtype=L.SIM_PARAMS)
+        bodies = np.ascontiguousarray(np.asarray(bodies, dtype=L.BODY).reshape(-1))
+        materials = np.ascontiguousarray(np.asarray(materials, dtype=L.BOUNDARY_COND).reshape(-1))
+        h = C.c_void_p()
+        _lib.check(_lib.load().slosh_data_create(pipeline.handle, _p(self._params), C.byref(st),
+                                                 int(particle_capacity), _p(bodies) if len(bodies) else None,
+                                                 len(bodies), _p(materials) if len(materials) else None,
+                                                 float(cell_width), int(grid_capacity), C.byref(h)))
+        self.handle = h
+        self.nbodies = len(bodies)
+        self.batched = scene_ids is not None
+
+    @classmethod
+    def from_scene(cls, pipeline, scene, **kw):
+        return cls(pipeline, scene.params, scene.particles, scene.bodies, scene.materials, scene.cell_width,
+                   scene.grid_capacity, **kw)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().slosh_data_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def close(self):
+        self.__del__()
+
+    @property
+    def n(self):
+        return int(_lib.load().slosh_num_particles(self.handle))
+
+    def _s(self, stream):
+        return stream.handle if stream else None
+
+    def set_sim_params(self, params):
+        self._params = np.array([params], dtype=L.SIM_PARAMS)
+        _lib.check(_lib.load().slosh_set_sim_params(self.handle, _p(self._params)))
+
+    def set_svd_fp64(self, enable):
+        _lib.check(_lib.load().slosh_set_svd_fp64(self.handle, 1 if enable else 0))
+
+    def check_errors(self, stream=None):
+        _lib.check(_lib.load().slosh_check_errors(self.handle, self._s(stream)))
+
+    def positions(self, stream=None):
+        out = np.zeros((self.n, 4), np.float32)
+        _lib.check(_lib.load().slosh_read_positions(self.handle, self._s(stream), _p(out)))
+        return out
+
+    def kinematics(self, stream=None):
+        out = np.zeros(self.n, L.KINEMATICS)
+        _lib.check(_lib.load().slosh_read_kinematics(self.handle, self._s(stream), _p(out)))
+        return out
+
+    def def_grad(self, stream=None):
+        out = np.zeros((self.n, 3, 4), np.float32)
+        _lib.check(_lib.load().slosh_read_def_grad(self.handle, self._s(stream), _p(out)))
+        return out
+
+    def models(self, stream=None):
+        out = np.zeros((self.n, 13), np.uint32)
+        _lib.check(_lib.load().slosh_read_models(self.handle, self._s(stream), _p(out)))
+        return out
+
+    def properties(self, stream=None):
+        out = np.zeros(self.n, L.PROPERTIES)
+        _lib.check(_lib.load().slosh_read_properties(self.handle, self._s(stream), _p(out)))
+        return out
+
+    def scene_ids(self, stream=None):
+        out = np.zeros(self.n, np.uint32)
+        _lib.check(_lib.load().slosh_read_scene_ids(self.handle, self._s(stream), _p(out)))
+        return out
+
+    def bodies(self, stream=None):
+        poses = np.zeros(self.nbodies, L.SIM3)
+        vels = np.zeros(self.nbodies, L.VELOCITY)
+        _lib.check(_lib.load().slosh_read_bodies(self.handle, self._s(stream), _p(poses), _p(vels)))
+        return poses, vels
+
+    def num_blocks(self, stream=None):
+        b = C.c_uint32()
+        _lib.check(_lib.load().slosh_num_blocks(self.handle, self._s(stream), C.byref(b)))
+        return b.value
+
+    def blocks(self, stream=None):
+        nb = self.num_blocks(stream)
+        out = np.zeros(nb, L.BLOCK_INFO)
+        w = C.c_uint32()
+        if nb:
+            _lib.check(_lib.load().slosh_read_blocks(self.handle, self._s(stream), _p(out), nb, C.byref(w)))
+        return out[:w.value] if nb else out
+
+    def block_scenes(self, stream=None):
+        nb = self.num_blocks(stream)
+        out = np.zeros(nb, np.uint32)
+        if nb:
+            _lib.check(_lib.load().slosh_read_block_scenes(self.handle, self._s(stream), _p(out), nb))
+        return out
+
+    def nodes(self, stream=None):
+        nb = self.num_blocks(stream)
+        out = np.zeros((nb * 64, 4), np.float32)
+        if nb:
+            _lib.check(_lib.load().slosh_read_nodes(self.handle, self._s(stream), _p(out), nb))
+        return out
+
+    def append(self, particles, scene_ids=None, stream=None):
+        st, keep = _particles_struct(particles, scene_ids)
+        _lib.check(_lib.load().slosh_append_particles(self.handle, self._s(stream), C.byref(st)))
+
+    def shift_remove(self, start, end, stream=None):
+        _lib.check(_lib.load().slosh_remove_particles(self.handle, self._s(stream), int(start), int(end)))
+
+    def remove_scene(self, scene_id, stream=None):
+        r = C.c_size_t()
+        _lib.check(_lib.load().slosh_remove_scene(self.handle, self._s(stream), int(scene_id), C.byref(r)))
+        return r.value
+
+    def write_kinematics(self, kin, stream=None):
+        kin = np.ascontiguousarray(kin, dtype=L.KINEMATICS)
+        _lib.check(_lib.load().slosh_write_kinematics(self.handle, self._s(stream), _p(kin)))
+
+    def device_view(self):
+        v = _lib.DeviceView()
+        _lib.check(_lib.load().slosh_device_view_get(self.handle, C.byref(v)))
+        return v
+
+
+def device_count():
+    n = C.c_int(0)
+    rc = _lib.load().slosh_device_count(C.byref(n))
+    return n.value if rc == 0 else 0

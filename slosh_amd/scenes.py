@@ -1,0 +1,214 @@
+"""Seeded scene generators for the benchmark configurations and the parity tests.
+
+RNG: SplitMix64 with ``u = (x >> 40) * 2**-24``; the seed of configuration k is
+``0x5105_0000 + k`` (batched scenes add the scene index). The generators are
+shared by the HIP path and by the CPU checker so both see identical inputs.
+
+Configurations (sizes as in the benchmark definition):
+
+* ``elastic_cube``   -- 32^3 lattice, spacing 1/128, cube of side 0.25 centred at
+  (0.5, 0.4, 0.5), jitter U(+-0.1 dx) with dx = 1/64, neo-Hookean E=1e5 nu=0.3,
+  rho=1000, r = dx/4, six sticky cuboid walls, dt=1e-4.
+* ``dam_break``      -- 2^20 particles on a 64 x 256 x 64 lattice at spacing 1/512
+  (column [0,0.125] x [0,0.5] x [0,0.125]), dx = 1/256, neo-Hookean with
+  lambda=1e5, mu=0 (weakly compressible, bulk modulus 1e5), rho=1000,
+  separating walls with zero friction, dt=2e-5.
+* ``sand_spheres``   -- 2^22 particles, dx=1/512: a Drucker-Prager sand bed
+  (friction angle 30 deg, E=3.5e5, nu=0.3, rho=1600) plus 64 neo-Hookean spheres
+  (E=2e5, nu=0.4, R=0.03, rho=1000) with velocities U[-0.5,0.5]^3 and eight
+  two-way rigid boxes of side 0.1 at seeded poses, dt=2e-5.
+* ``batched_scene``  -- one scene of the 256-scene batch: N log-uniform in
+  [2e3, 2e5], material uniform over NH / FC / DP, E log-uniform [1e4, 1e6],
+  nu U[0.2, 0.45], dx = 1/128, the elastic-cube layout scaled to N.
+"""
+import numpy as np
+
+from . import layouts as L
+from . import models as M
+
+SEED_BASE = 0x51050000
+_MASK = (1 << 64) - 1
+
+
+def splitmix64(seed, count):
+    """``count`` consecutive SplitMix64 outputs for ``seed`` (uint64 array)."""
+    with np.errstate(over="ignore"):
+        i = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed & _MASK) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed, count):
+    """U[0,1) doubles from the top 24 bits of each SplitMix64 output."""
+    return (splitmix64(seed, count) >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
+
+
+class Scene:
+    """Inputs of ``MpmData::new``: params, particles (reference SoA layouts),
+    bodies + per-body boundary conditions, cell width and grid capacity."""
+
+    def __init__(self, name, particles, dt, cell_width, grid_capacity, bodies, materials,
+                 gravity=(0.0, -9.81, 0.0), substeps=1):
+        self.name = name
+        self.particles = particles
+        self.params = np.zeros(1, dtype=L.SIM_PARAMS)[0]
+        self.params["gravity"] = gravity
+        self.params["dt"] = dt
+        self.cell_width = np.float32(cell_width)
+        self.grid_capacity = int(grid_capacity)
+        self.bodies = np.asarray(bodies, dtype=L.BODY).reshape(-1)
+        self.materials = np.asarray(materials, dtype=L.BOUNDARY_COND).reshape(-1)
+        self.substeps = substeps
+
+    @property
+    def n(self):
+        return self.particles["pos4"].shape[0]
+
+
+def box_walls(bc_type, friction=0.0, thickness=0.1):
+    """Six fixed cuboid colliders enclosing [0,1]^3 (fixed bodies: zero mass props)."""
+    t = thickness / 2
+    ext = 0.5 + thickness
+    specs = [((0.5, -t, 0.5), (ext, t, ext)), ((0.5, 1.0 + t, 0.5), (ext, t, ext)),
+             ((-t, 0.5, 0.5), (t, ext, ext)), ((1.0 + t, 0.5, 0.5), (t, ext, ext)),
+             ((0.5, 0.5, -t), (ext, ext, t)), ((0.5, 0.5, 1.0 + t), (ext, ext, t))]
+    bodies = [M.body(M.cuboid(he), c) for c, he in specs]
+    mats = [M.boundary_condition(bc_type, friction)] * 6
+    return bodies, mats
+
+
+def lattice(counts, spacing, origin, jitter, seed):
+    """Regular lattice (x fastest, then y, then z) with seeded uniform jitter."""
+    nx, ny, nz = counts
+    i = np.arange(nx, dtype=np.float64)
+    j = np.arange(ny, dtype=np.float64)
+    k = np.arange(nz, dtype=np.float64)
+    zz, yy, xx = np.meshgrid(k, j, i, indexing="ij")
+    pts = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1)
+    pts = (pts + 0.5) * spacing + np.asarray(origin, dtype=np.float64)
+    u = uniform(seed, pts.shape[0] * 3).reshape(-1, 3)
+    pts += (2.0 * u - 1.0) * jitter
+    return pts.astype(np.float32)
+
+
+def elastic_cube(n_side=32, seed=SEED_BASE + 0):
+    dx = 1.0 / 64
+    spacing = 1.0 / 128
+    side = n_side * spacing
+    origin = (0.5 - side / 2, 0.4 - side / 2, 0.5 - side / 2)
+    pos = lattice((n_side,) * 3, spacing, origin, 0.1 * dx, seed)
+    parts = M.particle_arrays(pos, np.float32(dx / 4), 1000.0, M.elastic_neo_hookean(1.0e5, 0.3))
+    bodies, mats = box_walls(M.STICK)
+    return Scene("elastic_cube", parts, 1.0e-4, dx, 1024, bodies, mats, substeps=100)
+
+
+def dam_break(counts=(64, 256, 64), seed=SEED_BASE + 1, grid_capacity=16384):
+    dx = 1.0 / 256
+    spacing = 1.0 / 512
+    pos = lattice(counts, spacing, (0.0, 0.0, 0.0), 0.1 * dx, seed)
+    model = M.elastic_from_lame(M.ELASTIC_NEO_HOOKEAN, 1.0e5, 0.0)
+    parts = M.particle_arrays(pos, np.float32(dx / 4), 1000.0, model)
+    bodies, mats = box_walls(M.SEPARATE, 0.0)
+    return Scene("dam_break", parts, 2.0e-5, dx, grid_capacity, bodies, mats, substeps=1000)
+
+
+def _ball_points(n, radius, center, seed):
+    u = uniform(seed, n * 3).reshape(-1, 3)
+    r = radius * np.cbrt(u[:, 0])
+    ct = 2.0 * u[:, 1] - 1.0
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    ph = 2.0 * np.pi * u[:, 2]
+    d = np.stack([st * np.cos(ph), ct, st * np.sin(ph)], axis=1)
+    return (np.asarray(center) + d * r[:, None]).astype(np.float32)
+
+
+def sand_spheres(n_sand=2097152, n_per_sphere=32768, seed=SEED_BASE + 2, grid_capacity=1 << 17):
+    dx = 1.0 / 512
+    u = uniform(seed, n_sand * 6).reshape(-1, 6)
+    lo = np.array([0.375, 0.0, 0.375])
+    ext = np.array([0.25, 0.03125, 0.25])
+    sand_pos = (lo + u[:, :3] * ext).astype(np.float32)
+    sand_vel = (u[:, 3:] - 0.5).astype(np.float32)
+    sand = M.particle_arrays(sand_pos, np.float32(dx / 4), 1600.0, M.sand(3.5e5, 0.3, 30.0), sand_vel)
+    parts = [sand]
+    nh = M.elastic_neo_hookean(2.0e5, 0.4)
+    for s in range(64):
+        i, j, k = s % 4, (s // 4) % 4, s // 16
+        c = (0.5 + (i - 1.5) * 0.07, 0.09 + j * 0.07, 0.5 + (k - 1.5) * 0.07)
+        sp = _ball_points(n_per_sphere, 0.03, c, seed * 131 + s + 1)
+        sv = (uniform(seed * 257 + s + 1, n_per_sphere * 3).reshape(-1, 3) - 0.5).astype(np.float32)
+        parts.append(M.particle_arrays(sp, np.float32(dx / 4), 1000.0, nh, sv))
+    particles = M.concat_particles(parts)
+    bodies, mats = box_walls(M.SEPARATE, 0.0)
+    ub = uniform(seed * 7 + 3, 8 * 6).reshape(8, 6)
+    for b in range(8):
+        t = (0.2 + 0.6 * ub[b, 0], 0.4 + 0.2 * ub[b, 1], 0.2 + 0.6 * ub[b, 2])
+        rot = M.quat_from_scaled_axis((ub[b, 3:] - 0.5) * 2.0)
+        bodies.append(M.body(M.cuboid((0.05, 0.05, 0.05)), t, rot, mass=1.0, inv_inertia_diag=(600.0,) * 3))
+        mats.append(M.boundary_condition(M.SEPARATE, 0.5))
+    return Scene("sand_spheres", particles, 2.0e-5, dx, grid_capacity, bodies, mats, substeps=500)
+
+
+def batched_scene(scene_index, seed_base=SEED_BASE + 3, n_override=None):
+    """One scene of the 256-scene batch (elastic-cube layout scaled to N)."""
+    seed = seed_base + scene_index
+    u = uniform(seed * 1009 + 17, 4)
+    n = int(np.exp(np.log(2.0e3) + u[0] * (np.log(2.0e5) - np.log(2.0e3)))) if n_override is None else n_override
+    mat = int(u[1] * 3.0) % 3
+    e = float(np.exp(np.log(1.0e4) + u[2] * (np.log(1.0e6) - np.log(1.0e4))))
+    nu = 0.2 + 0.25 * u[3]
+    dx = 1.0 / 128
+    side = int(np.ceil(n ** (1.0 / 3.0)))
+    spacing = dx / 2
+    ext = side * spacing
+    origin = (0.5 - ext / 2, 0.4 - ext / 2, 0.5 - ext / 2)
+    pos = lattice((side, side, side), spacing, origin, 0.1 * dx, seed)[:n]
+    model = [M.elastic_neo_hookean(e, nu), M.elastic(e, nu), M.sand(e, nu, 30.0)][mat]
+    parts = M.particle_arrays(pos, np.float32(dx / 4), 1000.0, model)
+    bodies, mats = box_walls(M.STICK)
+    return Scene(f"batched_{scene_index}", parts, 1.0e-4, dx, 4096, bodies, mats, substeps=200)
+
+
+def mixed_test_scene(n_side=20, seed=12345, moving_body=True):
+    """Parity scene that exercises every branch: all four material tags, nonzero
+    velocity gradients and affine matrices, perturbed deformation gradients, a
+    fixed particle, damping, a rotated cuboid, a ball and (optionally) a moving
+    two-way body so that the collision cache is exercised."""
+    dx = 1.0 / 64
+    spacing = dx / 2
+    side = n_side * spacing
+    origin = (0.5 - side / 2, 0.12, 0.5 - side / 2)
+    pos = lattice((n_side,) * 3, spacing, origin, 0.1 * dx, seed)
+    n = pos.shape[0]
+    u = uniform(seed + 1, n * 16).reshape(n, 16)
+    rel = pos.astype(np.float64) - np.array([0.5, 0.12 + side / 2, 0.5])
+    omega = np.array([0.3, 1.1, -0.4])
+    vel = np.cross(omega, rel) + np.array([0.05, -0.2, 0.1]) + (u[:, :3] - 0.5) * 0.05
+    tags = (np.arange(n) * 7 // 3) % 4
+    recs = np.stack([M.elastic(2.0e4, 0.3), M.elastic_neo_hookean(3.0e4, 0.25),
+                     M.sand(5.0e4, 0.3, 30.0), M.sand(4.0e4, 0.35, 25.0, neo_hookean=True)])
+    models = recs[tags]
+    parts = M.particle_arrays(pos, np.float32(dx / 4), 1000.0, models, vel.astype(np.float32))
+    aff = (u[:, 3:12].reshape(n, 3, 3) - 0.5) * 0.02
+    parts["kin"]["affine"] = L.mat_to_slang(aff)
+    f = np.eye(3)[None] + (u[:, 7:16].reshape(n, 3, 3) - 0.5) * 0.01
+    parts["def_grad"] = L.mat_to_slang(f)
+    parts["props"]["damping"][::5] = 0.5
+    parts["props"]["fixed"][::97] = 1
+    parts["kin"]["force_dt"][::3] = (0.0, 1.0e-7, 0.0)
+    bodies, mats = box_walls(M.SLIP)
+    mats[0] = M.boundary_condition(M.SEPARATE, 0.3)
+    mats[1] = M.boundary_condition(M.FRICTION_Z_UP, 0.2)
+    mats[2] = M.boundary_condition(M.STICK)
+    rot = M.quat_from_scaled_axis((0.2, 0.5, -0.3))
+    bodies.append(M.body(M.cuboid((0.04, 0.03, 0.05)), (0.5, 0.08, 0.5), rot))
+    mats.append(M.boundary_condition(M.SEPARATE, 0.1))
+    bodies.append(M.body(M.ball(0.03), (0.44, 0.2, 0.47)))
+    mats.append(M.boundary_condition(M.SLIP))
+    if moving_body:
+        bodies.append(M.body(M.cuboid((0.03, 0.03, 0.03)), (0.56, 0.2, 0.52), linvel=(-0.4, 0.1, 0.0),
+                             angvel=(0.0, 2.0, 1.0), mass=0.5, inv_inertia_diag=(900.0, 900.0, 900.0)))
+        mats.append(M.boundary_condition(M.SEPARATE, 0.2))
+    return Scene("mixed", parts, 1.0e-4, dx, 2048, bodies, mats, substeps=100)
