@@ -18,8 +18,9 @@
 //   k_count            per-particle hash lookup (dedup per wave) and per-(block,cell)
 //                      rank via one aggregated atomic per distinct key per wave
 //   k_block_prepare    per block: wave-level exclusive scan of its 64 cell counts,
-//                      block range allocation, +1/-1 neighbour and previous-step ids
-//   k_scatter          sorted slot = cell start + rank
+//                      the block total, +1/-1 neighbour and previous-step ids
+//   k_block_scan       exclusive scan of the block totals (one workgroup)
+//   k_scatter          sorted slot = block start + cell offset + rank
 #include "common.h"
 
 namespace {
@@ -88,17 +89,20 @@ __global__ __launch_bounds__(256) void k_touch_neighbors(uint32_t *__restrict__ 
                                                          uint32_t cap, uint32_t batched,
                                                          uint32_t *__restrict__ counters) {
     // touch_neighbor_blocks (sort.slang:75-92): only slots flagged primary by
-    // k_touch_primary expand; slots this kernel fills keep flag 0.
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += stride) {
+    // k_touch_primary expand; slots this kernel fills keep flag 0. One lane per
+    // (slot, neighbour offset) so the 7 insertions of a block run in parallel.
+    const uint64_t total = (uint64_t)cap * 8u;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+        const uint32_t s = (uint32_t)(t >> 3), o = (uint32_t)(t & 7u);
+        if (o == 0) continue;
         uint32_t k = keys[s];
         if (k == SLOSH_NONE || vals[s] != 1u) continue;
         int x, y, z;
         uint32_t scene;
         unpack_key(batched, k, &x, &y, &z, &scene);
-        for (int o = 1; o < 8; o++)
-            hash_insert(keys, vals, cap, pack_key(batched, scene, x + ASSOC[o][0], y + ASSOC[o][1], z + ASSOC[o][2]),
-                        false, counters);
+        hash_insert(keys, vals, cap, pack_key(batched, scene, x + ASSOC[o][0], y + ASSOC[o][1], z + ASSOC[o][2]),
+                    false, counters);
     }
 }
 
@@ -177,14 +181,16 @@ __global__ __launch_bounds__(256) void k_count(const float4 *__restrict__ pos, u
 }
 
 __global__ __launch_bounds__(256) void k_block_prepare(const uint32_t *__restrict__ cell_cnt,
-                                                       uint32_t *__restrict__ cell_start,
-                                                       uint32_t *__restrict__ blk_start, uint32_t *__restrict__ blk_cnt,
+                                                       uint32_t *__restrict__ cell_start, uint32_t *__restrict__ blk_cnt,
                                                        const uint32_t *__restrict__ hdr_key,
                                                        uint32_t *__restrict__ nbr, const uint32_t *__restrict__ keys,
                                                        const uint32_t *__restrict__ vals,
                                                        const uint32_t *__restrict__ prev_keys,
                                                        const uint32_t *__restrict__ prev_vals, uint32_t cap,
-                                                       uint32_t batched, uint32_t *__restrict__ counters) {
+                                                       uint32_t batched, const uint32_t *__restrict__ counters) {
+    // Per block (one wavefront): exclusive scan of the 64 cell counts (offsets
+    // relative to the block's first slot), the block total, and the neighbour
+    // table (+1 / -1 neighbours and the previous-substep header id).
     const uint32_t B = counters[CTR_NUM_BLOCKS];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t waves_per_grid = gridDim.x * (blockDim.x >> 6);
@@ -196,15 +202,8 @@ __global__ __launch_bounds__(256) void k_block_prepare(const uint32_t *__restric
             uint32_t y = __shfl_up(x, d);
             if ((int)lane >= d) x += y;
         }
-        uint32_t total = __shfl(x, 63);
-        uint32_t start = 0u;
-        if (lane == 0) start = atomicAdd(&counters[CTR_SORTED_ALLOC], total);
-        start = __shfl(start, 0);
-        cell_start[(size_t)hid * NODES_PER_BLOCK + lane] = start + x - c;
-        if (lane == 0) {
-            blk_start[hid] = start;
-            blk_cnt[hid] = total;
-        }
+        cell_start[(size_t)hid * NODES_PER_BLOCK + lane] = x - c;
+        if (lane == 63) blk_cnt[hid] = x;
         uint32_t k = hdr_key[hid];
         int bx, by, bz;
         uint32_t scene;
@@ -223,13 +222,59 @@ __global__ __launch_bounds__(256) void k_block_prepare(const uint32_t *__restric
     }
 }
 
+// Exclusive scan of the block totals in one workgroup of 1024 threads (4
+// consecutive blocks per thread per pass): blk_start[b] = sum of blk_cnt[< b].
+__global__ __launch_bounds__(1024) void k_block_scan(const uint32_t *__restrict__ blk_cnt,
+                                                     uint32_t *__restrict__ blk_start,
+                                                     uint32_t *__restrict__ counters) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < B; base += 4096u) {
+        const uint32_t idx = base + tid * 4u;
+        uint32_t v0 = idx < B ? blk_cnt[idx] : 0u;
+        uint32_t v1 = idx + 1 < B ? blk_cnt[idx + 1] : 0u;
+        uint32_t v2 = idx + 2 < B ? blk_cnt[idx + 2] : 0u;
+        uint32_t v3 = idx + 3 < B ? blk_cnt[idx + 3] : 0u;
+        const uint32_t t = v0 + v1 + v2 + v3;
+        uint32_t x = t;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t y = __shfl_up(x, d);
+            if ((int)lane >= d) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t w = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                uint32_t y = __shfl_up(w, d);
+                if ((int)lane >= d) w += y;
+            }
+            if (lane < 16) wsum[lane] = w;
+        }
+        __syncthreads();
+        const uint32_t e = carry + (wave > 0 ? wsum[wave - 1] : 0u) + x - t;
+        if (idx < B) blk_start[idx] = e;
+        if (idx + 1 < B) blk_start[idx + 1] = e + v0;
+        if (idx + 2 < B) blk_start[idx + 2] = e + v0 + v1;
+        if (idx + 3 < B) blk_start[idx + 3] = e + v0 + v1 + v2;
+        carry += wsum[15];
+        __syncthreads();
+    }
+    if (tid == 0) counters[CTR_SORTED_ALLOC] = carry;
+}
+
 __global__ __launch_bounds__(256) void k_scatter(const uint32_t *__restrict__ pslot, const uint32_t *__restrict__ prank,
-                                                 const uint32_t *__restrict__ cell_start, uint32_t *__restrict__ perm,
+                                                 const uint32_t *__restrict__ cell_start,
+                                                 const uint32_t *__restrict__ blk_start, uint32_t *__restrict__ perm,
                                                  uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t s = pslot[i];
-    if (s != SLOSH_NONE) perm[cell_start[s] + prank[i]] = i;
+    if (s != SLOSH_NONE) perm[blk_start[s / NODES_PER_BLOCK] + cell_start[s] + prank[i]] = i;
 }
 
 }  // namespace
@@ -253,17 +298,17 @@ hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n
     hipLaunchKernelGGL(k_reset_table, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, g.counters);
     if (n > 0) hipLaunchKernelGGL(k_touch_primary, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched, g.keys,
                                   g.vals, cap, g.counters);
-    hipLaunchKernelGGL(k_touch_neighbors, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, batched,
-                       g.counters);
+    hipLaunchKernelGGL(k_touch_neighbors, dim3(min_u(div_up((uint64_t)cap * 8u, 256), 8192)), dim3(256), 0, stream,
+                       g.keys, g.vals, cap, batched, g.counters);
     hipLaunchKernelGGL(k_compact, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, g.hdr_key, g.cell_cnt,
                        g.counters);
     if (n > 0) hipLaunchKernelGGL(k_count, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched, g.keys, g.vals,
                                   cap, g.cell_cnt, g.pslot, g.prank, g.counters);
     const uint32_t blk_grid = min_u(div_up(cap, 4), 4096);
-    hipLaunchKernelGGL(k_block_prepare, dim3(blk_grid), dim3(256), 0, stream, g.cell_cnt, g.cell_start, g.blk_start,
-                       g.blk_cnt, g.hdr_key, g.nbr, g.keys, g.vals, g.prev_keys, g.prev_vals, cap, batched,
-                       g.counters);
+    hipLaunchKernelGGL(k_block_prepare, dim3(blk_grid), dim3(256), 0, stream, g.cell_cnt, g.cell_start, g.blk_cnt,
+                       g.hdr_key, g.nbr, g.keys, g.vals, g.prev_keys, g.prev_vals, cap, batched, g.counters);
+    hipLaunchKernelGGL(k_block_scan, dim3(1), dim3(1024), 0, stream, g.blk_cnt, g.blk_start, g.counters);
     if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(part_grid), dim3(256), 0, stream, g.pslot, g.prank, g.cell_start,
-                                  g.perm, n);
+                                  g.blk_start, g.perm, n);
     return hipGetLastError();
 }

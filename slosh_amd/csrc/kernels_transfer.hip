@@ -2,12 +2,12 @@
 // integration of one substep on gfx950.
 //
 //   k_p2g            one workgroup per active block: lane = associated cell,
-//                    wave w takes particles w, w+4, ... of that cell and
-//                    scatters its 27 stencil contributions into a 6x6x6 LDS
-//                    tile with LDS float atomics; the tile is written once to
-//                    HBM (no global atomics). Same per-(node, particle)
-//                    arithmetic as p2g_scatter_style.slang:84-148; only the
-//                    summation order differs.
+//                    each wave accumulates its particles' 27 stencil
+//                    contributions in registers and merges them into a 6x6x6
+//                    LDS tile (no atomics, LDS or global); the tile is written
+//                    once to HBM. Same per-(node, particle) arithmetic as
+//                    p2g_scatter_style.slang:84-148; only the summation order
+//                    differs (and is fixed for a given particle order).
 //   k_grid_update    one wave per block: node momentum = own tile + the halo
 //                    parts of the 7 "-1" neighbours' tiles, then
 //                    grid_update.slang:38-163 (gravity, clamp, collision
@@ -594,31 +594,34 @@ __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 
 // ---------------------------------------------------------------------------
 // Kernels.
 
-__global__ __launch_bounds__(256) void k_p2g(StepParams P, const float4 *__restrict__ pos, const float4 *__restrict__ vm,
-                                             const float *__restrict__ C, const float *__restrict__ fdt, uint32_t pcap,
-                                             const uint32_t *__restrict__ perm, const uint32_t *__restrict__ cell_cnt,
-                                             const uint32_t *__restrict__ cell_start,
-                                             const uint32_t *__restrict__ hdr_key, float4 *__restrict__ tiles,
-                                             const uint32_t *__restrict__ counters) {
-    __shared__ float t_x[TILE_NODES], t_y[TILE_NODES], t_z[TILE_NODES], t_m[TILE_NODES];
+// P2G for one block per workgroup of WPB waves. Lane = associated cell of the
+// block; wave w accumulates particles w, w + WPB, ... of its cell into 27 x 4
+// registers (one per stencil node), then the waves merge their registers into
+// the 6x6x6 LDS tile one after the other, in 27 rounds in which the 64 lanes of
+// a wave touch 64 distinct nodes (plain read-modify-write; no LDS atomics).
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_p2g(StepParams P, const float4 *__restrict__ pos,
+                                                  const float4 *__restrict__ vm, const float *__restrict__ C,
+                                                  const float *__restrict__ fdt, uint32_t pcap,
+                                                  const uint32_t *__restrict__ perm,
+                                                  const uint32_t *__restrict__ cell_cnt,
+                                                  const uint32_t *__restrict__ cell_start,
+                                                  const uint32_t *__restrict__ blk_start,
+                                                  const uint32_t *__restrict__ hdr_key, float4 *__restrict__ tiles,
+                                                  const uint32_t *__restrict__ counters) {
+    __shared__ float4 tile[TILE_NODES];
     const uint32_t B = counters[CTR_NUM_BLOCKS];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
     const uint32_t tbase = lx + 6 * ly + 36 * lz;
     const float h = P.h, inv_h = P.inv_h;
     for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
-        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 256) {
-            t_x[j] = 0.0f;
-            t_y[j] = 0.0f;
-            t_z[j] = 0.0f;
-            t_m[j] = 0.0f;
-        }
-        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 64 * WPB) tile[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         int bx, by, bz;
         uint32_t scene;
         unpack_key(P.batched, hdr_key[hid], &bx, &by, &bz, &scene);
         const uint32_t cnt = cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane];
-        const uint32_t st = cell_start[(size_t)hid * NODES_PER_BLOCK + lane];
+        const uint32_t st = blk_start[hid] + cell_start[(size_t)hid * NODES_PER_BLOCK + lane];
         float nx[3], ny[3], nz[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -626,7 +629,10 @@ __global__ __launch_bounds__(256) void k_p2g(StepParams P, const float4 *__restr
             ny[k] = (float)(by * 4 + ly + k) * h;
             nz[k] = (float)(bz * 4 + lz + k) * h;
         }
-        for (uint32_t j = wave; j < cnt; j += 4) {
+        float acc[27][4];
+#pragma unroll
+        for (int k = 0; k < 27; k++) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.0f;
+        for (uint32_t j = wave; j < cnt; j += WPB) {
             const uint32_t p = perm[st + j];
             const float4 ps = pos[p];
             const float4 v = vm[p];
@@ -655,20 +661,43 @@ __global__ __launch_bounds__(256) void k_p2g(StepParams P, const float4 *__restr
                 for (int ky = 0; ky < 3; ky++)
 #pragma unroll
                     for (int kx = 0; kx < 3; kx++) {
+                        const int k = kx + 3 * ky + 9 * kz;
                         const float w = wx[kx] * wy[ky] * wz[kz];
                         const float ax = dx[kx] * c[0] + dy[ky] * c[3] + dz[kz] * c[6] + mx;
                         const float ay = dx[kx] * c[1] + dy[ky] * c[4] + dz[kz] * c[7] + my;
                         const float az = dx[kx] * c[2] + dy[ky] * c[5] + dz[kz] * c[8] + mz;
-                        const uint32_t node = tbase + kx + 6 * ky + 36 * kz;
-                        atomicAdd(&t_x[node], ax * w);
-                        atomicAdd(&t_y[node], ay * w);
-                        atomicAdd(&t_z[node], az * w);
-                        atomicAdd(&t_m[node], v.w * w);
+                        acc[k][0] += ax * w;
+                        acc[k][1] += ay * w;
+                        acc[k][2] += az * w;
+                        acc[k][3] += v.w * w;
                     }
         }
         __syncthreads();
-        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 256)
-            tiles[(size_t)hid * TILE_NODES + j] = make_float4(t_x[j], t_y[j], t_z[j], t_m[j]);
+#pragma unroll
+        for (int ww = 0; ww < WPB; ww++) {
+            if ((int)wave == ww) {
+#pragma unroll
+                for (int kz = 0; kz < 3; kz++)
+#pragma unroll
+                    for (int ky = 0; ky < 3; ky++)
+#pragma unroll
+                        for (int kx = 0; kx < 3; kx++) {
+                            const int k = kx + 3 * ky + 9 * kz;
+                            const uint32_t node = tbase + kx + 6 * ky + 36 * kz;
+                            float4 t = tile[node];
+                            t.x += acc[k][0];
+                            t.y += acc[k][1];
+                            t.z += acc[k][2];
+                            t.w += acc[k][3];
+                            tile[node] = t;
+                            // The next round's nodes overlap other lanes' nodes of this
+                            // round: complete this round's LDS writes first.
+                            __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                        }
+            }
+            __syncthreads();
+        }
+        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 64 * WPB) tiles[(size_t)hid * TILE_NODES + j] = tile[j];
         __syncthreads();
     }
 }
@@ -1013,8 +1042,8 @@ uint32_t block_grid_size(uint32_t cap, uint32_t per_wg) { return min_t(div_up_t(
 
 hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t pcap, const TransferBuffers &t,
                       hipStream_t stream) {
-    hipLaunchKernelGGL(k_p2g, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, pb.pos, pb.vm, pb.C, t.fdt, pcap,
-                       t.perm, t.cell_cnt, t.cell_start, t.hdr_key, t.tiles, t.counters);
+    hipLaunchKernelGGL(k_p2g<2>, dim3(block_grid_size(P.cap, 1)), dim3(128), 0, stream, P, pb.pos, pb.vm, pb.C, t.fdt,
+                       pcap, t.perm, t.cell_cnt, t.cell_start, t.blk_start, t.hdr_key, t.tiles, t.counters);
     return hipGetLastError();
 }
 
