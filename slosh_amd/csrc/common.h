@@ -15,7 +15,8 @@
 #define NODES_PER_BLOCK 64
 #define TILE_NODES 216
 
-// Counter slots in the per-data device counter array.
+// Counter slots of one substep's counter set (two sets alternate with the hash
+// tables so a substep can prepare the next one's).
 #define CTR_NUM_BLOCKS 0
 #define CTR_SORTED_ALLOC 1
 #define CTR_ERRORS 2
@@ -44,6 +45,9 @@ struct StepParams {
     uint32_t coll_cache;  // use the per-node collision cache (moving colliders)
     uint32_t cur;         // ping-pong index of the current hash table / cache
     uint32_t svd_f64;     // SVD-based stress / plasticity in FP64 (1) or FP32 (0)
+    uint32_t tag_mask;    // material tags present (bit t = tag t; bit 4 = any other tag)
+    uint32_t uniform;     // every particle has the same static record: use u_* below
+    float4 u_prop, u_model, u_dp0, u_dp1;
 };
 
 // ---------------------------------------------------------------------------
@@ -120,3 +124,54 @@ __device__ inline uint32_t hash_find(const uint32_t *__restrict__ keys, const ui
 // Wave-level helpers (64-lane wavefronts).
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 __device__ inline uint64_t lanemask_lt() { return __lanemask_lt(); }
+
+// Open-addressing insertion with a strong CAS (insertion_index, grid.slang:118-174).
+// `primary` marks the slot as a primary block for the neighbour expansion.
+__device__ inline void hash_insert(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals, uint32_t cap,
+                                   uint32_t key, bool primary, uint32_t *__restrict__ counters) {
+    uint32_t slot = hash_key(key) & (cap - 1u);
+    for (uint32_t k = 0; k < cap; k++) {
+        uint32_t old = atomicCAS(&keys[slot], SLOSH_NONE, key);
+        if (old == SLOSH_NONE) {
+            if (primary) vals[slot] = 1u;
+            return;
+        }
+        if (old == key) return;
+        slot = (slot + 1u) & (cap - 1u);
+    }
+    atomicOr(&counters[CTR_ERRORS], ERR_HASH_FULL);
+}
+
+// Primary-block key of a particle (x, y, z, w = scene << 1 | enabled bits).
+__device__ inline uint32_t particle_key(const float4 &p, float h, uint32_t batched, uint32_t *cell) {
+    int bx, by, bz;
+    assoc_block_cell(h, p.x, p.y, p.z, &bx, &by, &bz, cell);
+    uint32_t scene = __float_as_uint(p.w) >> 1;
+    return pack_key(batched, scene, bx, by, bz);
+}
+
+// Runs of equal keys over consecutive lanes of a wavefront (particle storage is
+// sorted by block and cell, so most wavefronts hold a few long runs). Returns
+// the mask of run heads; invalid lanes are heads of their own (ignored) runs.
+// Every lane of the wavefront must call it.
+__device__ inline uint64_t run_heads(uint32_t key, bool valid) {
+    const uint32_t lane = lane_id();
+    const uint32_t prev = __shfl_up(key, 1);
+    const int prev_valid = __shfl_up((int)valid, 1);
+    const bool head = !valid || lane == 0 || !prev_valid || prev != key;
+    return __ballot(head);
+}
+
+// Lane index of the head of this lane's run.
+__device__ inline int run_head_of(uint64_t heads) {
+    const uint64_t le = heads & (lanemask_lt() | (1ull << lane_id()));
+    return 63 - __clzll((long long)le);
+}
+
+// Length of the run starting at this (head) lane.
+__device__ inline uint32_t run_length(uint64_t heads) {
+    const uint32_t lane = lane_id();
+    const uint64_t above = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1ull);
+    const uint32_t next = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : 64u;
+    return next - lane;
+}

@@ -10,13 +10,14 @@
 // Kernel sequence (one launch each, all grid-stride / upper-bound grids so the
 // sequence is graph-capturable without reading counts back):
 //   k_reset_table      hash slots <- NONE, per-step counters <- 0
-//   k_touch_primary    each particle inserts its primary block; one CAS per
-//                      distinct key per wavefront (wave-level dedup)
+//   k_touch_primary    each particle inserts its primary block; one CAS per run
+//                      of equal keys in a wavefront, all runs concurrently
 //   k_touch_neighbors  each primary slot inserts its 7 +1 neighbours
 //   k_compact          occupied slots -> dense header ids (ballot + popcount,
 //                      one atomic per wavefront); zeroes the block's cell counters
-//   k_count            per-particle hash lookup (dedup per wave) and per-(block,cell)
-//                      rank via one aggregated atomic per distinct key per wave
+//   k_count            per-particle hash lookup (one per run of equal block keys)
+//                      and per-(block, cell) rank via one aggregated atomic per
+//                      run of equal (block, cell) keys
 //   k_block_prepare    per block: wave-level exclusive scan of its 64 cell counts,
 //                      the block total, +1/-1 neighbour and previous-step ids
 //   k_block_scan       exclusive scan of the block totals (one workgroup)
@@ -27,29 +28,6 @@ namespace {
 
 constexpr int ASSOC[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1},
                              {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
-
-__device__ inline void hash_insert(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals, uint32_t cap,
-                                   uint32_t key, bool primary, uint32_t *__restrict__ counters) {
-    // insertion_index (grid.slang:118-174). HIP's atomicCAS is a strong CAS.
-    uint32_t slot = hash_key(key) & (cap - 1u);
-    for (uint32_t k = 0; k < cap; k++) {
-        uint32_t old = atomicCAS(&keys[slot], SLOSH_NONE, key);
-        if (old == SLOSH_NONE) {
-            if (primary) vals[slot] = 1u;
-            return;
-        }
-        if (old == key) return;
-        slot = (slot + 1u) & (cap - 1u);
-    }
-    atomicOr(&counters[CTR_ERRORS], ERR_HASH_FULL);
-}
-
-__device__ inline uint32_t particle_key(const float4 &p, float h, uint32_t batched, uint32_t *cell) {
-    int bx, by, bz;
-    assoc_block_cell(h, p.x, p.y, p.z, &bx, &by, &bz, cell);
-    uint32_t scene = __float_as_uint(p.w) >> 1;
-    return pack_key(batched, scene, bx, by, bz);
-}
 
 __global__ __launch_bounds__(256) void k_reset_table(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                      uint32_t cap, uint32_t *__restrict__ counters) {
@@ -69,20 +47,12 @@ __global__ __launch_bounds__(256) void k_touch_primary(const float4 *__restrict_
                                                        uint32_t *__restrict__ vals, uint32_t cap,
                                                        uint32_t *__restrict__ counters) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = i < n;
+    const bool valid = i < n;
     uint32_t key = 0u, cell;
-    if (active) key = particle_key(pos[i], h, batched, &cell);
-    // One insertion per distinct key of the wavefront.
-    while (true) {
-        uint64_t m = __ballot(active);
-        if (m == 0) break;
-        int leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t lk = __shfl(key, leader);
-        if (active && key == lk) {
-            if ((int)lane_id() == leader) hash_insert(keys, vals, cap, lk, true, counters);
-            active = false;
-        }
-    }
+    if (valid) key = particle_key(pos[i], h, batched, &cell);
+    // One insertion per run of equal keys; all runs of the wavefront insert concurrently.
+    const uint64_t heads = run_heads(key, valid);
+    if (valid && ((heads >> lane_id()) & 1ull)) hash_insert(keys, vals, cap, key, true, counters);
 }
 
 __global__ __launch_bounds__(256) void k_touch_neighbors(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
@@ -135,48 +105,27 @@ __global__ __launch_bounds__(256) void k_count(const float4 *__restrict__ pos, u
                                                uint32_t *__restrict__ pslot, uint32_t *__restrict__ prank,
                                                uint32_t *__restrict__ counters) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = i < n;
+    const bool valid = i < n;
+    const uint32_t lane = lane_id();
     uint32_t cell = 0u, key = 0u;
     if (valid) key = particle_key(pos[i], h, batched, &cell);
-    // Hash lookup, once per distinct block key of the wavefront.
-    uint32_t hid = SLOSH_NONE;
-    bool need = valid;
-    while (true) {
-        uint64_t m = __ballot(need);
-        if (m == 0) break;
-        int leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t lk = __shfl(key, leader);
-        uint32_t hv = 0u;
-        if ((int)lane_id() == leader) hv = hash_find(keys, vals, cap, lk);
-        hv = __shfl(hv, leader);
-        if (need && key == lk) {
-            hid = hv;
-            need = false;
-        }
-    }
-    uint32_t sk = (valid && hid != SLOSH_NONE) ? hid * NODES_PER_BLOCK + cell : SLOSH_NONE;
+    // Hash lookup by the head of each run of equal block keys, all runs at once.
+    const uint64_t bheads = run_heads(key, valid);
+    uint32_t hv = SLOSH_NONE;
+    if (valid && ((bheads >> lane) & 1ull)) hv = hash_find(keys, vals, cap, key);
+    const uint32_t hid = __shfl(hv, run_head_of(bheads));
+    const bool ok = valid && hid != SLOSH_NONE;
     if (valid && hid == SLOSH_NONE) atomicOr(&counters[CTR_ERRORS], ERR_LOOKUP);
-    // Per-(block, cell) rank: one aggregated atomic per distinct key of the wavefront.
-    bool act = sk != SLOSH_NONE;
-    uint32_t rank = 0u;
-    while (true) {
-        uint64_t m = __ballot(act);
-        if (m == 0) break;
-        int leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t lk = __shfl(sk, leader);
-        bool match = act && sk == lk;
-        uint64_t mm = __ballot(match);
-        uint32_t b = 0u;
-        if ((int)lane_id() == leader) b = atomicAdd(&cell_cnt[lk], (uint32_t)__popcll(mm));
-        b = __shfl(b, leader);
-        if (match) {
-            rank = b + (uint32_t)__popcll(mm & lanemask_lt());
-            act = false;
-        }
-    }
+    const uint32_t sk = ok ? hid * NODES_PER_BLOCK + cell : SLOSH_NONE;
+    // Per-(block, cell) rank: one atomic per run of equal (block, cell) keys.
+    const uint64_t cheads = run_heads(sk, ok);
+    uint32_t base = 0u;
+    if (ok && ((cheads >> lane) & 1ull)) base = atomicAdd(&cell_cnt[sk], run_length(cheads));
+    const int head = run_head_of(cheads);
+    base = __shfl(base, head);
     if (valid) {
         pslot[i] = sk;
-        prank[i] = rank;
+        prank[i] = base + (lane - (uint32_t)head);
     }
 }
 
@@ -291,13 +240,17 @@ struct GridBuffers {
 static inline uint32_t div_up(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
 static inline uint32_t min_u(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
+// `primed`: the previous substep already reset this table and inserted every
+// particle's primary block (fused into its P2G and G2P kernels).
 hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n, float h, uint32_t batched,
-                             uint32_t cap, hipStream_t stream) {
+                             uint32_t cap, bool primed, hipStream_t stream) {
     const uint32_t slot_grid = min_u(div_up(cap, 256), 2048);
     const uint32_t part_grid = div_up(n, 256);
-    hipLaunchKernelGGL(k_reset_table, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, g.counters);
-    if (n > 0) hipLaunchKernelGGL(k_touch_primary, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched, g.keys,
-                                  g.vals, cap, g.counters);
+    if (!primed) {
+        hipLaunchKernelGGL(k_reset_table, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, g.counters);
+        if (n > 0) hipLaunchKernelGGL(k_touch_primary, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched,
+                                      g.keys, g.vals, cap, g.counters);
+    }
     hipLaunchKernelGGL(k_touch_neighbors, dim3(min_u(div_up((uint64_t)cap * 8u, 256), 8192)), dim3(256), 0, stream,
                        g.keys, g.vals, cap, batched, g.counters);
     hipLaunchKernelGGL(k_compact, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, g.hdr_key, g.cell_cnt,

@@ -526,7 +526,10 @@ __device__ inline void g2p_particle(const float4 *__restrict__ sh, float h, floa
 }
 
 // particle_update.slang:32-160 for one particle. `aff` holds the velocity
-// gradient on entry and the APIC affine matrix on exit.
+// gradient on entry and the APIC affine matrix on exit. MODELS is the set of
+// material tags the kernel instance handles (bit t = tag t), so scenes without
+// SVD-based models do not carry that code's register footprint.
+template <uint32_t MODELS>
 __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 &v, float &mass, uint32_t &enabled,
                                        float *aff, float *F, float4 prop, float4 model, const PStatic &st,
                                        uint32_t oid, float *dps) {
@@ -555,14 +558,14 @@ __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 
     // DefaultParticleModel::update, default.slang:51-92
     uint32_t tag = __float_as_uint(model.x);
     float stress[9];
-    if (tag == 1u) {
+    if ((MODELS & 2u) && tag == 1u) {
         nh_stress(model.y, model.z, F, stress);
-    } else if (tag == 0u) {
+    } else if ((MODELS & 1u) && tag == 0u) {
         if (P.svd_f64) fc_stress_f64(model.y, model.z, F, stress);
         else fc_stress(model.y, model.z, F, stress);
-    } else if (tag == 2u || tag == 3u) {
-        float4 hp = st.dp0[oid];
-        float4 pl = st.dp1[oid];
+    } else if (((MODELS & 4u) && tag == 2u) || ((MODELS & 8u) && tag == 3u)) {
+        float4 hp = P.uniform ? P.u_dp0 : st.dp0[oid];
+        float4 pl = P.uniform ? P.u_dp1 : st.dp1[oid];
         if (P.svd_f64) dp_project_f64(dps, hp, pl.x, pl.y, F);
         else dp_project(dps, hp, pl.x, pl.y, F);
         if (tag == 3u) nh_stress(model.y, model.z, F, stress);
@@ -594,110 +597,180 @@ __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 
 // ---------------------------------------------------------------------------
 // Kernels.
 
-// P2G for one block per workgroup of WPB waves. Lane = associated cell of the
-// block; wave w accumulates particles w, w + WPB, ... of its cell into 27 x 4
-// registers (one per stencil node), then the waves merge their registers into
-// the 6x6x6 LDS tile one after the other, in 27 rounds in which the 64 lanes of
-// a wave touch 64 distinct nodes (plain read-modify-write; no LDS atomics).
-template <int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_p2g(StepParams P, const float4 *__restrict__ pos,
-                                                  const float4 *__restrict__ vm, const float *__restrict__ C,
-                                                  const float *__restrict__ fdt, uint32_t pcap,
-                                                  const uint32_t *__restrict__ perm,
-                                                  const uint32_t *__restrict__ cell_cnt,
-                                                  const uint32_t *__restrict__ cell_start,
-                                                  const uint32_t *__restrict__ blk_start,
-                                                  const uint32_t *__restrict__ hdr_key, float4 *__restrict__ tiles,
-                                                  const uint32_t *__restrict__ counters) {
+// P2G for one block per workgroup of 3 waves.
+// Phase 1 (per chunk of P2G_CHUNK sorted particles of the block): each thread
+// loads one particle with coalesced reads (particle storage follows the block
+// order of the previous substep) and stages position, mass, momentum and the
+// APIC matrix in LDS.
+// Phase 2: lane = associated cell of the block, wave = z offset kz of the
+// stencil; each lane accumulates, over the particles of its cell in the chunk,
+// the contributions to the 9 nodes (cell + (kx, ky, kz)) in 36 registers.
+// Phase 3: the waves merge their registers into the 6x6x6 LDS tile one after
+// the other, in 9 rounds in which the 64 lanes of a wave touch 64 distinct nodes
+// (plain read-modify-write; no LDS or global atomics). The summation order is
+// fixed by the particle order of each cell.
+// Cells hold ~8 particles, so lanes read particles ~8 apart: pad one word every
+// 8 to spread them over the LDS banks.
+__device__ inline uint32_t p2g_slot(uint32_t q) { return q + (q >> 3); }
+
+// Next substep's hash table, reset here and filled with primary blocks by the
+// fused G2P / particle-update kernel (replaces k_reset_table + k_touch_primary
+// of the next substep).
+struct NextTable {
+    uint32_t *keys, *vals, *counters;
+    uint32_t prime;
+};
+
+template <int P2G_CHUNK>
+__global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const float4 *__restrict__ pos,
+                                             const float4 *__restrict__ vm, const float *__restrict__ C,
+                                             const float *__restrict__ fdt, uint32_t pcap,
+                                             const uint32_t *__restrict__ perm,
+                                             const uint32_t *__restrict__ cell_cnt,
+                                             const uint32_t *__restrict__ cell_start,
+                                             const uint32_t *__restrict__ blk_start,
+                                             const uint32_t *__restrict__ blk_cnt,
+                                             const uint32_t *__restrict__ hdr_key, float4 *__restrict__ tiles,
+                                             const uint32_t *__restrict__ counters) {
     __shared__ float4 tile[TILE_NODES];
+    constexpr int P2G_LDS = P2G_CHUNK + P2G_CHUNK / 8;
+    constexpr int R = (P2G_CHUNK + 191) / 192;
+    __shared__ float sp[16][P2G_LDS];  // x, y, z, m, mom x, y, z, C[9]
+    if (nt.prime) {
+        for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < P.cap; s += gridDim.x * blockDim.x) {
+            nt.keys[s] = SLOSH_NONE;
+            nt.vals[s] = 0u;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            nt.counters[CTR_NUM_BLOCKS] = 0u;
+            nt.counters[CTR_SORTED_ALLOC] = 0u;
+        }
+    }
     const uint32_t B = counters[CTR_NUM_BLOCKS];
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t kz = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
-    const uint32_t tbase = lx + 6 * ly + 36 * lz;
+    const uint32_t tbase = lx + 6 * ly + 36 * (lz + (int)kz);
     const float h = P.h, inv_h = P.inv_h;
     for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
-        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 64 * WPB) tile[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 192) tile[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         int bx, by, bz;
         uint32_t scene;
         unpack_key(P.batched, hdr_key[hid], &bx, &by, &bz, &scene);
-        const uint32_t cnt = cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane];
-        const uint32_t st = blk_start[hid] + cell_start[(size_t)hid * NODES_PER_BLOCK + lane];
-        float nx[3], ny[3], nz[3];
+        const uint32_t b0 = blk_start[hid], nb = blk_cnt[hid];
+        const uint32_t cs = cell_start[(size_t)hid * NODES_PER_BLOCK + lane];
+        const uint32_t ce = cs + cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane];
+        float nx[3], ny[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             nx[k] = (float)(bx * 4 + lx + k) * h;
             ny[k] = (float)(by * 4 + ly + k) * h;
-            nz[k] = (float)(bz * 4 + lz + k) * h;
         }
-        float acc[27][4];
+        const float nz = (float)(bz * 4 + lz + (int)kz) * h;
+        float acc[9][4];
 #pragma unroll
-        for (int k = 0; k < 27; k++) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.0f;
-        for (uint32_t j = wave; j < cnt; j += WPB) {
-            const uint32_t p = perm[st + j];
-            const float4 ps = pos[p];
-            const float4 v = vm[p];
-            float c[9];
+        for (int k = 0; k < 9; k++) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.0f;
+        for (uint32_t chunk = 0; chunk < nb; chunk += P2G_CHUNK) {
+            __syncthreads();
+            // All of this thread's loads are issued before any is consumed.
+            uint32_t pp[R];
 #pragma unroll
-            for (int k = 0; k < 9; k++) c[k] = C[(size_t)k * pcap + p];
-            float mx = v.x * v.w, my = v.y * v.w, mz = v.z * v.w;
-            if (P.has_force) {
-                mx += fdt[p];
-                my += fdt[(size_t)pcap + p];
-                mz += fdt[2 * (size_t)pcap + p];
+            for (int r = 0; r < R; r++) {
+                const uint32_t t = threadIdx.x + 192u * r;
+                pp[r] = (t < (uint32_t)P2G_CHUNK && chunk + t < nb) ? perm[b0 + chunk + t] : SLOSH_NONE;
             }
-            float dx[3], dy[3], dz[3], wx[3], wy[3], wz[3];
+            float4 ps[R], vv[R];
+            float cc[R][9];
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                dx[k] = nx[k] - ps.x;
-                dy[k] = ny[k] - ps.y;
-                dz[k] = nz[k] - ps.z;
-                wx[k] = k_eval(dx[k] * inv_h);
-                wy[k] = k_eval(dy[k] * inv_h);
-                wz[k] = k_eval(dz[k] * inv_h);
+            for (int r = 0; r < R; r++) {
+                if (pp[r] != SLOSH_NONE) {
+                    ps[r] = pos[pp[r]];
+                    vv[r] = vm[pp[r]];
+#pragma unroll
+                    for (int k = 0; k < 9; k++) cc[r][k] = C[(size_t)k * pcap + pp[r]];
+                }
             }
 #pragma unroll
-            for (int kz = 0; kz < 3; kz++)
+            for (int r = 0; r < R; r++) {
+                if (pp[r] == SLOSH_NONE) continue;
+                const uint32_t p = pp[r];
+                float mx = vv[r].x * vv[r].w, my = vv[r].y * vv[r].w, mz = vv[r].z * vv[r].w;
+                if (P.has_force) {
+                    mx += fdt[p];
+                    my += fdt[(size_t)pcap + p];
+                    mz += fdt[2 * (size_t)pcap + p];
+                }
+                const uint32_t q = p2g_slot(threadIdx.x + 192u * r);
+                sp[0][q] = ps[r].x;
+                sp[1][q] = ps[r].y;
+                sp[2][q] = ps[r].z;
+                sp[3][q] = vv[r].w;
+                sp[4][q] = mx;
+                sp[5][q] = my;
+                sp[6][q] = mz;
+#pragma unroll
+                for (int k = 0; k < 9; k++) sp[7 + k][q] = cc[r][k];
+            }
+            __syncthreads();
+            const uint32_t j0 = cs > chunk ? cs : chunk;
+            const uint32_t j1 = ce < chunk + P2G_CHUNK ? ce : chunk + P2G_CHUNK;
+            for (uint32_t j = j0; j < j1; j++) {
+                const uint32_t q = p2g_slot(j - chunk);
+                const float px = sp[0][q], py = sp[1][q], pz = sp[2][q], m = sp[3][q];
+                float c[9];
+#pragma unroll
+                for (int k = 0; k < 9; k++) c[k] = sp[7 + k][q];
+                float dx[3], dy[3], wx[3], wy[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    dx[k] = nx[k] - px;
+                    dy[k] = ny[k] - py;
+                    wx[k] = k_eval(dx[k] * inv_h);
+                    wy[k] = k_eval(dy[k] * inv_h);
+                }
+                const float dz = nz - pz;
+                const float wz = k_eval(dz * inv_h);
+                // mul(dpt, affine) + momentum: the z term is shared by the 9 nodes of the slab.
+                const float bx_ = dz * c[6] + sp[4][q], by_ = dz * c[7] + sp[5][q], bz_ = dz * c[8] + sp[6][q];
 #pragma unroll
                 for (int ky = 0; ky < 3; ky++)
 #pragma unroll
                     for (int kx = 0; kx < 3; kx++) {
-                        const int k = kx + 3 * ky + 9 * kz;
-                        const float w = wx[kx] * wy[ky] * wz[kz];
-                        const float ax = dx[kx] * c[0] + dy[ky] * c[3] + dz[kz] * c[6] + mx;
-                        const float ay = dx[kx] * c[1] + dy[ky] * c[4] + dz[kz] * c[7] + my;
-                        const float az = dx[kx] * c[2] + dy[ky] * c[5] + dz[kz] * c[8] + mz;
+                        const int k = kx + 3 * ky;
+                        const float w = wx[kx] * wy[ky] * wz;
+                        const float ax = dx[kx] * c[0] + dy[ky] * c[3] + bx_;
+                        const float ay = dx[kx] * c[1] + dy[ky] * c[4] + by_;
+                        const float az = dx[kx] * c[2] + dy[ky] * c[5] + bz_;
                         acc[k][0] += ax * w;
                         acc[k][1] += ay * w;
                         acc[k][2] += az * w;
-                        acc[k][3] += v.w * w;
+                        acc[k][3] += m * w;
                     }
+            }
         }
         __syncthreads();
 #pragma unroll
-        for (int ww = 0; ww < WPB; ww++) {
-            if ((int)wave == ww) {
+        for (int ww = 0; ww < 3; ww++) {
+            if ((int)kz == ww) {
 #pragma unroll
-                for (int kz = 0; kz < 3; kz++)
+                for (int ky = 0; ky < 3; ky++)
 #pragma unroll
-                    for (int ky = 0; ky < 3; ky++)
-#pragma unroll
-                        for (int kx = 0; kx < 3; kx++) {
-                            const int k = kx + 3 * ky + 9 * kz;
-                            const uint32_t node = tbase + kx + 6 * ky + 36 * kz;
-                            float4 t = tile[node];
-                            t.x += acc[k][0];
-                            t.y += acc[k][1];
-                            t.z += acc[k][2];
-                            t.w += acc[k][3];
-                            tile[node] = t;
-                            // The next round's nodes overlap other lanes' nodes of this
-                            // round: complete this round's LDS writes first.
-                            __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-                        }
+                    for (int kx = 0; kx < 3; kx++) {
+                        const int k = kx + 3 * ky;
+                        const uint32_t node = tbase + kx + 6 * ky;
+                        float4 t = tile[node];
+                        t.x += acc[k][0];
+                        t.y += acc[k][1];
+                        t.z += acc[k][2];
+                        t.w += acc[k][3];
+                        tile[node] = t;
+                        // The next round's nodes overlap other lanes' nodes of this
+                        // round: complete this round's LDS writes first.
+                        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                    }
             }
             __syncthreads();
         }
-        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 64 * WPB) tiles[(size_t)hid * TILE_NODES + j] = tile[j];
+        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 192) tiles[(size_t)hid * TILE_NODES + j] = tile[j];
         __syncthreads();
     }
 }
@@ -780,147 +853,6 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
     }
 }
 
-__device__ inline void load_halo(float4 *__restrict__ sh, const float4 *__restrict__ nodes,
-                                 const uint32_t *__restrict__ nbr, uint32_t hid) {
-    for (uint32_t j = threadIdx.x; j < TILE_NODES; j += blockDim.x) {
-        uint32_t sx = j % 6u, sy = (j / 6u) % 6u, sz = j / 36u;
-        uint32_t ox = sx >= 4u, oy = sy >= 4u, oz = sz >= 4u;
-        uint32_t oi = ox * 4u + oy * 2u + oz;
-        uint32_t src = oi == 0u ? hid : nbr[(size_t)hid * NBR_STRIDE + oi - 1u];
-        uint32_t node = (sx - 4u * ox) + 4u * (sy - 4u * oy) + 16u * (sz - 4u * oz);
-        sh[j] = src != SLOSH_NONE ? nodes[(size_t)src * NODES_PER_BLOCK + node] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
-                                                    float *__restrict__ fdt, const uint32_t *__restrict__ perm,
-                                                    const uint32_t *__restrict__ blk_start,
-                                                    const uint32_t *__restrict__ blk_cnt,
-                                                    const uint32_t *__restrict__ nbr, const float4 *__restrict__ nodes,
-                                                    const uint32_t *__restrict__ counters) {
-    __shared__ float4 sh[TILE_NODES];
-    const uint32_t B = counters[CTR_NUM_BLOCKS];
-    for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
-        load_halo(sh, nodes, nbr, hid);
-        __syncthreads();
-        const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
-        for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
-            const uint32_t slot = s0 + j;
-            const uint32_t p = perm[slot];
-            const float4 ps = in.pos[p];
-            const uint32_t wbits = __float_as_uint(ps.w);
-            uint32_t enabled = wbits & 1u;
-            float3 pos = make_float3(ps.x, ps.y, ps.z);
-            float3 v;
-            float aff[9], F[9];
-            float vgd;
-            g2p_particle(sh, P.h, pos, enabled != 0u, &v, aff, &vgd);
-            float mass = in.vm[p].w;
-#pragma unroll
-            for (int k = 0; k < 9; k++) F[k] = in.F[(size_t)k * pcap + p];
-            const uint32_t oid = in.ids[p];
-            const float4 prop = st.prop[oid];
-            const float4 model = st.model[oid];
-            float dps[3] = {0.0f, 0.0f, 0.0f};
-            if (P.has_sand) {
-#pragma unroll
-                for (int k = 0; k < 3; k++) dps[k] = in.dps[(size_t)k * pcap + p];
-            }
-            update_particle(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
-            out.pos[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
-            out.vm[slot] = make_float4(v.x, v.y, v.z, mass);
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                out.C[(size_t)k * pcap + slot] = aff[k];
-                out.F[(size_t)k * pcap + slot] = F[k];
-            }
-            out.vgd[slot] = vgd;
-            out.ids[slot] = oid;
-            if (P.has_sand) {
-#pragma unroll
-                for (int k = 0; k < 3; k++) out.dps[(size_t)k * pcap + slot] = dps[k];
-            }
-            if (P.has_force) {
-                fdt[slot] = 0.0f;
-                fdt[(size_t)pcap + slot] = 0.0f;
-                fdt[2 * (size_t)pcap + slot] = 0.0f;
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// G2P only, in place (hook path): writes velocity, velocity gradient, vel_grad_det.
-__global__ __launch_bounds__(256) void k_g2p(StepParams P, PBuf buf, uint32_t pcap, const uint32_t *__restrict__ perm,
-                                             const uint32_t *__restrict__ blk_start,
-                                             const uint32_t *__restrict__ blk_cnt, const uint32_t *__restrict__ nbr,
-                                             const float4 *__restrict__ nodes, const uint32_t *__restrict__ counters) {
-    __shared__ float4 sh[TILE_NODES];
-    const uint32_t B = counters[CTR_NUM_BLOCKS];
-    for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
-        load_halo(sh, nodes, nbr, hid);
-        __syncthreads();
-        const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
-        for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
-            const uint32_t p = perm[s0 + j];
-            const float4 ps = buf.pos[p];
-            float3 v;
-            float g[9];
-            float vgd;
-            g2p_particle(sh, P.h, make_float3(ps.x, ps.y, ps.z), (__float_as_uint(ps.w) & 1u) != 0u, &v, g, &vgd);
-            float4 o = buf.vm[p];
-            buf.vm[p] = make_float4(v.x, v.y, v.z, o.w);
-#pragma unroll
-            for (int k = 0; k < 9; k++) buf.C[(size_t)k * pcap + p] = g[k];
-            buf.vgd[p] = vgd;
-        }
-        __syncthreads();
-    }
-}
-
-// Particle update only, in place (hook path), over every particle.
-__global__ __launch_bounds__(256) void k_particle_update(StepParams P, PBuf buf, PStatic st, uint32_t pcap,
-                                                         float *__restrict__ fdt) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P.n) return;
-    const float4 ps = buf.pos[p];
-    const uint32_t wbits = __float_as_uint(ps.w);
-    uint32_t enabled = wbits & 1u;
-    float3 pos = make_float3(ps.x, ps.y, ps.z);
-    float4 vmv = buf.vm[p];
-    float3 v = make_float3(vmv.x, vmv.y, vmv.z);
-    float mass = vmv.w;
-    float aff[9], F[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        aff[k] = buf.C[(size_t)k * pcap + p];
-        F[k] = buf.F[(size_t)k * pcap + p];
-    }
-    const uint32_t oid = buf.ids[p];
-    float dps[3] = {0.0f, 0.0f, 0.0f};
-    if (P.has_sand) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) dps[k] = buf.dps[(size_t)k * pcap + p];
-    }
-    update_particle(P, pos, v, mass, enabled, aff, F, st.prop[oid], st.model[oid], st, oid, dps);
-    buf.pos[p] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
-    buf.vm[p] = make_float4(v.x, v.y, v.z, mass);
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        buf.C[(size_t)k * pcap + p] = aff[k];
-        buf.F[(size_t)k * pcap + p] = F[k];
-    }
-    if (P.has_sand) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) buf.dps[(size_t)k * pcap + p] = dps[k];
-    }
-    if (P.has_force) {
-        fdt[p] = 0.0f;
-        fdt[(size_t)pcap + p] = 0.0f;
-        fdt[2 * (size_t)pcap + p] = 0.0f;
-    }
-}
-
 struct Bodies {
     float4 *poses;        // 2 per body
     float4 *vels;         // 2 per body: linear, angular
@@ -932,10 +864,8 @@ struct Bodies {
 
 __device__ inline float len3(float3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
 
-// rigid_impulses.slang:70-121 with body.slang:63-130
-__global__ void k_rigid_update(StepParams P, Bodies b) {
-    const uint32_t id = threadIdx.x;
-    if (id >= b.count) return;
+// rigid_impulses.slang:70-121 with body.slang:63-130, for body `id`.
+__device__ void integrate_body(const StepParams &P, const Bodies &b, uint32_t id) {
     const float dt = P.dt;
     float3 lin, ang;
     lin.x = (float)b.impulses[6 * id + 0] / 1.0e5f;
@@ -988,6 +918,176 @@ __global__ void k_rigid_update(StepParams P, Bodies b) {
     b.vels[2 * id + 1] = make_float4(na.x, na.y, na.z, 0.0f);
 }
 
+
+__global__ void k_rigid_update(StepParams P, Bodies b) {
+    if (threadIdx.x < b.count) integrate_body(P, b, threadIdx.x);
+}
+
+__device__ inline void load_halo(float4 *__restrict__ sh, const float4 *__restrict__ nodes,
+                                 const uint32_t *__restrict__ nbr, uint32_t hid) {
+    for (uint32_t j = threadIdx.x; j < TILE_NODES; j += blockDim.x) {
+        uint32_t sx = j % 6u, sy = (j / 6u) % 6u, sz = j / 36u;
+        uint32_t ox = sx >= 4u, oy = sy >= 4u, oz = sz >= 4u;
+        uint32_t oi = ox * 4u + oy * 2u + oz;
+        uint32_t src = oi == 0u ? hid : nbr[(size_t)hid * NBR_STRIDE + oi - 1u];
+        uint32_t node = (sx - 4u * ox) + 4u * (sy - 4u * oy) + 16u * (sz - 4u * oz);
+        sh[j] = src != SLOSH_NONE ? nodes[(size_t)src * NODES_PER_BLOCK + node] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+template <uint32_t MODELS>
+__global__ __launch_bounds__(256) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
+                                                    float *__restrict__ fdt, const uint32_t *__restrict__ perm,
+                                                    const uint32_t *__restrict__ blk_start,
+                                                    const uint32_t *__restrict__ blk_cnt,
+                                                    const uint32_t *__restrict__ nbr, const float4 *__restrict__ nodes,
+                                                    const uint32_t *__restrict__ counters, NextTable nt, Bodies bodies) {
+    __shared__ float4 sh[TILE_NODES];
+    // integrate_bodies: reads no particle or grid state (the impulses it applies
+    // are only written by the host), so it runs alongside the particle update.
+    if (blockIdx.x == 0 && threadIdx.x < bodies.count) integrate_body(P, bodies, threadIdx.x);
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
+        load_halo(sh, nodes, nbr, hid);
+        __syncthreads();
+        const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
+        // Every lane of the workgroup runs the same number of iterations so the
+        // wave-level run detection of the next table's insertion sees all lanes.
+        for (uint32_t jb = 0; jb < cnt; jb += blockDim.x) {
+            const uint32_t j = jb + threadIdx.x;
+            const bool valid = j < cnt;
+            uint32_t next_key = 0u;
+            if (valid) {
+            const uint32_t slot = s0 + j;
+            const uint32_t p = perm[slot];
+            const float4 ps = in.pos[p];
+            const uint32_t wbits = __float_as_uint(ps.w);
+            uint32_t enabled = wbits & 1u;
+            float3 pos = make_float3(ps.x, ps.y, ps.z);
+            float3 v;
+            float aff[9], F[9];
+            float vgd;
+            g2p_particle(sh, P.h, pos, enabled != 0u, &v, aff, &vgd);
+            float mass = in.vm[p].w;
+#pragma unroll
+            for (int k = 0; k < 9; k++) F[k] = in.F[(size_t)k * pcap + p];
+            const uint32_t oid = in.ids[p];
+            const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
+            const float4 model = P.uniform ? P.u_model : st.model[oid];
+            float dps[3] = {0.0f, 0.0f, 0.0f};
+            if (P.has_sand) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) dps[k] = in.dps[(size_t)k * pcap + p];
+            }
+            update_particle<MODELS>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
+            out.pos[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
+            out.vm[slot] = make_float4(v.x, v.y, v.z, mass);
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                out.C[(size_t)k * pcap + slot] = aff[k];
+                out.F[(size_t)k * pcap + slot] = F[k];
+            }
+            out.vgd[slot] = vgd;
+            out.ids[slot] = oid;
+            if (P.has_sand) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) out.dps[(size_t)k * pcap + slot] = dps[k];
+            }
+            if (P.has_force) {
+                fdt[slot] = 0.0f;
+                fdt[(size_t)pcap + slot] = 0.0f;
+                fdt[2 * (size_t)pcap + slot] = 0.0f;
+            }
+            if (nt.prime) {
+                uint32_t cell;
+                next_key = particle_key(make_float4(pos.x, pos.y, pos.z, __uint_as_float(wbits)), P.h, P.batched, &cell);
+            }
+            }
+            if (nt.prime) {
+                // touch_primary_blocks of the next substep (sort.slang:50-67), one
+                // insertion per run of equal keys.
+                const uint64_t heads = run_heads(next_key, valid);
+                if (valid && ((heads >> lane_id()) & 1ull)) hash_insert(nt.keys, nt.vals, P.cap, next_key, true, nt.counters);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// G2P only, in place (hook path): writes velocity, velocity gradient, vel_grad_det.
+__global__ __launch_bounds__(256) void k_g2p(StepParams P, PBuf buf, uint32_t pcap, const uint32_t *__restrict__ perm,
+                                             const uint32_t *__restrict__ blk_start,
+                                             const uint32_t *__restrict__ blk_cnt, const uint32_t *__restrict__ nbr,
+                                             const float4 *__restrict__ nodes, const uint32_t *__restrict__ counters) {
+    __shared__ float4 sh[TILE_NODES];
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
+        load_halo(sh, nodes, nbr, hid);
+        __syncthreads();
+        const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
+        for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+            const uint32_t p = perm[s0 + j];
+            const float4 ps = buf.pos[p];
+            float3 v;
+            float g[9];
+            float vgd;
+            g2p_particle(sh, P.h, make_float3(ps.x, ps.y, ps.z), (__float_as_uint(ps.w) & 1u) != 0u, &v, g, &vgd);
+            float4 o = buf.vm[p];
+            buf.vm[p] = make_float4(v.x, v.y, v.z, o.w);
+#pragma unroll
+            for (int k = 0; k < 9; k++) buf.C[(size_t)k * pcap + p] = g[k];
+            buf.vgd[p] = vgd;
+        }
+        __syncthreads();
+    }
+}
+
+// Particle update only, in place (hook path), over every particle.
+template <uint32_t MODELS>
+__global__ __launch_bounds__(256) void k_particle_update(StepParams P, PBuf buf, PStatic st, uint32_t pcap,
+                                                         float *__restrict__ fdt) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    const float4 ps = buf.pos[p];
+    const uint32_t wbits = __float_as_uint(ps.w);
+    uint32_t enabled = wbits & 1u;
+    float3 pos = make_float3(ps.x, ps.y, ps.z);
+    float4 vmv = buf.vm[p];
+    float3 v = make_float3(vmv.x, vmv.y, vmv.z);
+    float mass = vmv.w;
+    float aff[9], F[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        aff[k] = buf.C[(size_t)k * pcap + p];
+        F[k] = buf.F[(size_t)k * pcap + p];
+    }
+    const uint32_t oid = buf.ids[p];
+    float dps[3] = {0.0f, 0.0f, 0.0f};
+    if (P.has_sand) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) dps[k] = buf.dps[(size_t)k * pcap + p];
+    }
+    const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
+    const float4 model = P.uniform ? P.u_model : st.model[oid];
+    update_particle<MODELS>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
+    buf.pos[p] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
+    buf.vm[p] = make_float4(v.x, v.y, v.z, mass);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        buf.C[(size_t)k * pcap + p] = aff[k];
+        buf.F[(size_t)k * pcap + p] = F[k];
+    }
+    if (P.has_sand) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) buf.dps[(size_t)k * pcap + p] = dps[k];
+    }
+    if (P.has_force) {
+        fdt[p] = 0.0f;
+        fdt[(size_t)pcap + p] = 0.0f;
+        fdt[2 * (size_t)pcap + p] = 0.0f;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1023,6 +1123,11 @@ struct BodyBuffers {
     uint32_t count;
 };
 
+struct NextTableBuffers {
+    uint32_t *keys, *vals, *counters;
+    uint32_t prime;
+};
+
 static inline uint32_t div_up_t(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
 static inline uint32_t min_t(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
@@ -1041,9 +1146,15 @@ static inline PBuf to_pbuf(const ParticleBuffers &b) {
 uint32_t block_grid_size(uint32_t cap, uint32_t per_wg) { return min_t(div_up_t(cap, per_wg), 8192); }
 
 hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t pcap, const TransferBuffers &t,
-                      hipStream_t stream) {
-    hipLaunchKernelGGL(k_p2g<2>, dim3(block_grid_size(P.cap, 1)), dim3(128), 0, stream, P, pb.pos, pb.vm, pb.C, t.fdt,
-                       pcap, t.perm, t.cell_cnt, t.cell_start, t.blk_start, t.hdr_key, t.tiles, t.counters);
+                      const NextTableBuffers &next, hipStream_t stream) {
+    NextTable nt;
+    nt.keys = next.keys;
+    nt.vals = next.vals;
+    nt.counters = next.counters;
+    nt.prime = next.prime;
+    hipLaunchKernelGGL(k_p2g<512>, dim3(block_grid_size(P.cap, 1)), dim3(192), 0, stream, P, nt, pb.pos, pb.vm, pb.C,
+                       t.fdt, pcap, t.perm, t.cell_cnt, t.cell_start, t.blk_start, t.blk_cnt, t.hdr_key, t.tiles,
+                       t.counters);
     return hipGetLastError();
 }
 
@@ -1061,14 +1172,33 @@ hipError_t launch_grid_update(const StepParams &P, const TransferBuffers &t, con
 }
 
 hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, const ParticleBuffers &out,
-                             const StaticBuffers &s, uint32_t pcap, const TransferBuffers &t, hipStream_t stream) {
+                             const StaticBuffers &s, uint32_t pcap, const TransferBuffers &t,
+                             const NextTableBuffers &next, const BodyBuffers &bb, hipStream_t stream) {
+    NextTable nt;
+    nt.keys = next.keys;
+    nt.vals = next.vals;
+    nt.counters = next.counters;
+    nt.prime = next.prime;
+    Bodies b;
+    b.poses = bb.poses;
+    b.vels = bb.vels;
+    b.mprops = bb.mprops;
+    b.local_com = bb.local_com;
+    b.impulses = bb.impulses;
+    b.count = bb.count;
     PStatic st;
     st.prop = s.prop;
     st.model = s.model;
     st.dp0 = s.dp0;
     st.dp1 = s.dp1;
-    hipLaunchKernelGGL(k_g2p_update, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(in),
-                       to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, t.nodes, t.counters);
+    if (P.tag_mask == 2u)  // neo-Hookean only
+        hipLaunchKernelGGL(k_g2p_update<2u>, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(in),
+                           to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, t.nodes, t.counters, nt,
+                           b);
+    else
+        hipLaunchKernelGGL(k_g2p_update<15u>, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(in),
+                           to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, t.nodes, t.counters, nt,
+                           b);
     return hipGetLastError();
 }
 
@@ -1087,7 +1217,12 @@ hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b,
     st.model = s.model;
     st.dp0 = s.dp0;
     st.dp1 = s.dp1;
-    hipLaunchKernelGGL(k_particle_update, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st, pcap, fdt);
+    if (P.tag_mask == 2u)
+        hipLaunchKernelGGL(k_particle_update<2u>, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st,
+                           pcap, fdt);
+    else
+        hipLaunchKernelGGL(k_particle_update<15u>, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st,
+                           pcap, fdt);
     return hipGetLastError();
 }
 

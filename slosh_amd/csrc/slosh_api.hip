@@ -38,7 +38,7 @@ struct GridBuffers {
     uint32_t *pslot, *prank, *perm, *counters;
 };
 hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n, float h, uint32_t batched,
-                             uint32_t cap, hipStream_t stream);
+                             uint32_t cap, bool primed, hipStream_t stream);
 struct TransferBuffers {
     const uint32_t *perm, *cell_cnt, *cell_start, *blk_start, *blk_cnt, *hdr_key, *nbr, *counters;
     float4 *tiles, *nodes;
@@ -64,12 +64,17 @@ struct BodyBuffers {
     int *impulses;
     uint32_t count;
 };
+struct NextTableBuffers {
+    uint32_t *keys, *vals, *counters;
+    uint32_t prime;
+};
 hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t pcap, const TransferBuffers &t,
-                      hipStream_t stream);
+                      const NextTableBuffers &next, hipStream_t stream);
 hipError_t launch_grid_update(const StepParams &P, const TransferBuffers &t, const ColliderBuffers &c, uint32_t mode,
                               uint32_t do_update, hipStream_t stream);
 hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, const ParticleBuffers &out,
-                             const StaticBuffers &s, uint32_t pcap, const TransferBuffers &t, hipStream_t stream);
+                             const StaticBuffers &s, uint32_t pcap, const TransferBuffers &t,
+                             const NextTableBuffers &next, const BodyBuffers &bb, hipStream_t stream);
 hipError_t launch_g2p(const StepParams &P, const ParticleBuffers &b, uint32_t pcap, const TransferBuffers &t,
                       hipStream_t stream);
 hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
@@ -134,8 +139,14 @@ struct slosh_data {
     uint32_t coll_cache = 0;
     uint32_t coll_prev_valid = 0;
     uint32_t svd_f64 = 1;
+    uint32_t tag_mask = 0;  // material tags present (bit t = tag t, bit 4 = other)
+    uint32_t uniform = 0;   // all particles share one static material record
+    float4 u_prop{}, u_model{}, u_dp0{}, u_dp1{};
     int cur_p = 0;  // current particle buffer
     int cur_g = 0;  // hash table / collision cache written by the last substep
+    // The table of the next substep was reset and filled with every particle's
+    // primary block by the last substep's fused kernels.
+    bool next_primed = false;
     PBufs pb[2];
     float *fdt = nullptr;
     float4 *s_prop = nullptr, *s_model = nullptr, *s_dp0 = nullptr, *s_dp1 = nullptr;
@@ -191,7 +202,11 @@ static uint32_t f2u(float f) {
     return x;
 }
 
+// Two alternating sets of per-substep counters, one per hash table.
+static uint32_t *ctr_set(slosh_data *d, int g) { return d->counters + 8 * g; }
+
 static void invalidate_graphs(slosh_data *d) {
+    d->next_primed = false;
     for (int i = 0; i < 4; i++)
         if (d->graph[i]) {
             (void)hipGraphExecDestroy(d->graph[i]);
@@ -247,6 +262,39 @@ static void free_particle_buffers(slosh_data *d) {
 
 static bool model_is_sand(const slosh_particle_model &m) { return m.words[0] == 2u || m.words[0] == 3u; }
 
+// Static per-particle records as the kernels read them: prop (V0, damping, fixed,
+// radius), model (tag, elastic lambda, mu, cfl), dp0 (h0..h3), dp1 (plastic lambda, mu).
+static void static_records(const slosh_particle_properties &pr, const slosh_particle_model &m, float4 *prop,
+                           float4 *model, float4 *dp0, float4 *dp1);
+
+// Sets d->uniform when every particle has the same static material record (the
+// kernels then take it from their arguments instead of gathering it).
+static void update_uniform(slosh_data *d) {
+    d->uniform = 0;
+    const size_t n = d->models_host.size();
+    if (n == 0) return;
+    float4 p0, m0, a0, b0;
+    static_records(d->props_host[0], d->models_host[0], &p0, &m0, &a0, &b0);
+    for (size_t i = 1; i < n; i++) {
+        float4 p, m, a, b;
+        static_records(d->props_host[i], d->models_host[i], &p, &m, &a, &b);
+        if (std::memcmp(&p, &p0, 12) != 0 || std::memcmp(&m, &m0, 16) != 0 || std::memcmp(&a, &a0, 16) != 0 ||
+            std::memcmp(&b, &b0, 8) != 0)
+            return;
+    }
+    d->uniform = 1;
+    d->u_prop = p0;
+    d->u_model = m0;
+    d->u_dp0 = a0;
+    d->u_dp1 = b0;
+}
+
+static uint32_t tag_mask_of(const std::vector<slosh_particle_model> &models) {
+    uint32_t mask = 0;
+    for (const auto &m : models) mask |= m.words[0] < 4u ? (1u << m.words[0]) : 16u;
+    return mask;
+}
+
 // Host-side staging of particles in the internal layout (SoA, slot order).
 struct Staging {
     size_t n = 0;
@@ -296,26 +344,30 @@ static int stage_particles(const slosh_particles *p, uint32_t id_base, Staging &
             s.fdt[(size_t)c * n + i] = k.force_dt[c];
             if (k.force_dt[c] != 0.0f) *any_force = true;
         }
-        s.prop[4 * i + 0] = pr.init_volume;
-        s.prop[4 * i + 1] = pr.damping;
-        s.prop[4 * i + 2] = u2f(pr.fixed);
-        s.prop[4 * i + 3] = pr.init_radius;
-        const uint32_t tag = m.words[0];
-        s.model[4 * i + 0] = u2f(tag);
-        if (tag == 0u || tag == 1u) {
-            for (int c = 0; c < 3; c++) s.model[4 * i + 1 + c] = u2f(m.words[1 + c]);
-        } else if (tag == 2u || tag == 3u) {
+        static_records(pr, m, reinterpret_cast<float4 *>(&s.prop[4 * i]), reinterpret_cast<float4 *>(&s.model[4 * i]),
+                       reinterpret_cast<float4 *>(&s.dp0[4 * i]), reinterpret_cast<float4 *>(&s.dp1[4 * i]));
+        if (model_is_sand(m)) {
             *any_sand = true;
-            for (int c = 0; c < 3; c++) {
-                s.model[4 * i + 1 + c] = u2f(m.words[10 + c]);
-                s.dps[(size_t)c * n + i] = u2f(m.words[1 + c]);
-            }
-            for (int c = 0; c < 4; c++) s.dp0[4 * i + c] = u2f(m.words[4 + c]);
-            s.dp1[4 * i + 0] = u2f(m.words[8]);
-            s.dp1[4 * i + 1] = u2f(m.words[9]);
+            for (int c = 0; c < 3; c++) s.dps[(size_t)c * n + i] = u2f(m.words[1 + c]);
         }
     }
     return SLOSH_OK;
+}
+
+static void static_records(const slosh_particle_properties &pr, const slosh_particle_model &m, float4 *prop,
+                           float4 *model, float4 *dp0, float4 *dp1) {
+    *prop = make_float4(pr.init_volume, pr.damping, u2f(pr.fixed), pr.init_radius);
+    const uint32_t tag = m.words[0];
+    *model = make_float4(u2f(tag), 0.0f, 0.0f, 0.0f);
+    *dp0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    *dp1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (tag == 0u || tag == 1u) {
+        *model = make_float4(u2f(tag), u2f(m.words[1]), u2f(m.words[2]), u2f(m.words[3]));
+    } else if (tag == 2u || tag == 3u) {
+        *model = make_float4(u2f(tag), u2f(m.words[10]), u2f(m.words[11]), u2f(m.words[12]));
+        *dp0 = make_float4(u2f(m.words[4]), u2f(m.words[5]), u2f(m.words[6]), u2f(m.words[7]));
+        *dp1 = make_float4(u2f(m.words[8]), u2f(m.words[9]), 0.0f, 0.0f);
+    }
 }
 
 // Uploads staged particles into slots [slot0, slot0+n) of buffer `b` (ids starting at id0).
@@ -401,6 +453,12 @@ static StepParams make_params(const slosh_data *d) {
     P.coll_cache = d->coll_cache;
     P.cur = (uint32_t)d->cur_g;
     P.svd_f64 = d->svd_f64;
+    P.tag_mask = d->tag_mask;
+    P.uniform = d->uniform;
+    P.u_prop = d->u_prop;
+    P.u_model = d->u_model;
+    P.u_dp0 = d->u_dp0;
+    P.u_dp1 = d->u_dp1;
     return P;
 }
 
@@ -480,10 +538,20 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     g.pslot = d->pslot;
     g.prank = d->prank;
     g.perm = d->perm;
-    g.counters = d->counters;
+    g.counters = ctr_set(d, cur_g);
+
+    // The fused path (no hooks, built-in P2G and particle update) prepares the
+    // next substep's table inside this substep's kernels.
+    const bool prime = !any_hook && K.builtin_transfers && K.particles_update;
+    NextTableBuffers next;
+    next.keys = d->keys[prev_g];
+    next.vals = d->vals[prev_g];
+    next.counters = ctr_set(d, prev_g);
+    next.prime = prime ? 1u : 0u;
 
     record(ts, PASS_GRID_SORT, st, true);
-    HIP_TRY(launch_grid_build(g, d->pb[d->cur_p].pos, n, d->h, d->batched, d->cap, st));
+    HIP_TRY(launch_grid_build(g, d->pb[d->cur_p].pos, n, d->h, d->batched, d->cap, d->next_primed, st));
+    d->next_primed = false;
     record(ts, PASS_GRID_SORT, st, false);
     if (hooks && hooks->after_particle_sort && hooks->after_particle_sort(d, st, hs) != 0)
         return fail(SLOSH_E_HOOK, "after_particle_sort hook failed");
@@ -496,7 +564,7 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     t.blk_cnt = d->blk_cnt;
     t.hdr_key = d->hdr_key;
     t.nbr = d->nbr;
-    t.counters = d->counters;
+    t.counters = ctr_set(d, cur_g);
     t.tiles = d->tiles;
     t.nodes = d->nodes;
     t.coll_prev = (d->coll_cache && d->coll_prev_valid) ? (void *)d->coll[prev_g] : nullptr;
@@ -511,7 +579,7 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
         if (!K.builtin_transfers)
             return fail(SLOSH_E_NO_KERNELS, "pipeline built without builtin transfer kernels; a hook must handle run_p2g");
         record(ts, PASS_P2G, st, true);
-        HIP_TRY(launch_p2g(P, pbufs(d, d->cur_p), (uint32_t)d->pcap, t, st));
+        HIP_TRY(launch_p2g(P, pbufs(d, d->cur_p), (uint32_t)d->pcap, t, next, st));
         record(ts, PASS_P2G, st, false);
     }
     if (hooks && hooks->after_p2g && hooks->after_p2g(d, st, hs) != 0) return fail(SLOSH_E_HOOK, "after_p2g hook failed");
@@ -531,6 +599,14 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     if (hooks && hooks->after_grid_update && hooks->after_grid_update(d, st, hs) != 0)
         return fail(SLOSH_E_HOOK, "after_grid_update hook failed");
 
+    if (p2g_by_hook) next.prime = 0u;  // the table of the next substep was not reset
+    BodyBuffers bb;
+    bb.poses = d->poses;
+    bb.vels = d->vels;
+    bb.mprops = d->mprops;
+    bb.local_com = d->local_com;
+    bb.impulses = d->impulses;
+    bb.count = d->nbodies;
     handled = 0;
     if (hooks && hooks->run_g2p && hooks->run_g2p(d, st, hs, &handled) != 0)
         return fail(SLOSH_E_HOOK, "run_g2p hook failed");
@@ -540,10 +616,14 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
         return fail(SLOSH_E_NO_KERNELS, "pipeline built without builtin transfer kernels; a hook must handle run_g2p");
     if (!split) {
         // Fused G2P + particle update; writes the next particle buffer in sorted order.
+        // Fused G2P + particle update; writes the next particle buffer in sorted
+        // order and (fused) integrates the rigid bodies.
         record(ts, PASS_G2P, st, true);
-        HIP_TRY(launch_g2p_update(P, pbufs(d, d->cur_p), pbufs(d, d->cur_p ^ 1), sbufs(d), (uint32_t)d->pcap, t, st));
+        HIP_TRY(launch_g2p_update(P, pbufs(d, d->cur_p), pbufs(d, d->cur_p ^ 1), sbufs(d), (uint32_t)d->pcap, t, next,
+                                  bb, st));
         record(ts, PASS_G2P, st, false);
         d->cur_p ^= 1;
+        d->next_primed = next.prime != 0u;
     } else {
         if (!g2p_by_hook) {
             record(ts, PASS_G2P, st, true);
@@ -562,14 +642,7 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     if (hooks && hooks->after_particles_update && hooks->after_particles_update(d, st, hs) != 0)
         return fail(SLOSH_E_HOOK, "after_particles_update hook failed");
 
-    if (d->nbodies > 0) {
-        BodyBuffers bb;
-        bb.poses = d->poses;
-        bb.vels = d->vels;
-        bb.mprops = d->mprops;
-        bb.local_com = d->local_com;
-        bb.impulses = d->impulses;
-        bb.count = d->nbodies;
+    if (split && d->nbodies > 0) {
         record(ts, PASS_INTEGRATE_BODIES, st, true);
         HIP_TRY(launch_rigid_update(P, bb, st));
         record(ts, PASS_INTEGRATE_BODIES, st, false);
@@ -705,9 +778,11 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     d->has_force = any_force ? 1u : 0u;
     TRY(alloc_particle_buffers(d, d->pcap, d->has_sand != 0));
     d->models_host.assign(particles->models, particles->models + d->n);
+    d->tag_mask = tag_mask_of(d->models_host);
     d->props_host.assign(particles->properties, particles->properties + d->n);
     d->scene_host.resize(d->n);
     for (size_t i = 0; i < d->n; i++) d->scene_host[i] = particles->scene_ids ? particles->scene_ids[i] : 0u;
+    update_uniform(d);
     HIP_TRY(hipStreamCreateWithFlags(&d->capture_stream, hipStreamNonBlocking));
     TRY(upload_staging(d, s, 0, 0, 0, d->capture_stream));
     const size_t cap = d->cap;
@@ -723,8 +798,8 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     TRY(dalloc(&d->cell_start, cap * NODES_PER_BLOCK));
     TRY(dalloc(&d->blk_start, cap));
     TRY(dalloc(&d->blk_cnt, cap));
-    TRY(dalloc(&d->counters, 8));
-    HIP_TRY(hipMemset(d->counters, 0, 32));
+    TRY(dalloc(&d->counters, 16));
+    HIP_TRY(hipMemset(d->counters, 0, 64));
     TRY(dalloc(&d->tiles, cap * TILE_NODES));
     TRY(dalloc(&d->nodes, cap * NODES_PER_BLOCK));
     d->nbodies = (uint32_t)num_bodies;
@@ -788,7 +863,7 @@ int slosh_launch_steps_graph(slosh_pipeline *pipeline, slosh_data *d, slosh_stre
         // The substep alternates particle buffers and hash tables, so a pair of
         // substeps returns to the same parity; one graph per starting parity.
         const int slot = (d->cur_p & 1) | ((d->cur_g & 1) << 1);
-        if (d->has_force || (d->coll_cache && !d->coll_prev_valid)) {
+        if (d->has_force || (d->coll_cache && !d->coll_prev_valid) || !d->next_primed) {
             TRY(run_step(pipeline, d, st, nullptr, nullptr, nullptr));
             count--;
             continue;
@@ -796,6 +871,7 @@ int slosh_launch_steps_graph(slosh_pipeline *pipeline, slosh_data *d, slosh_stre
         if (!d->graph[slot]) {
             const int save_p = d->cur_p, save_g = d->cur_g;
             const uint32_t save_force = d->has_force;
+            const bool save_primed = d->next_primed;
             HIP_TRY(hipStreamBeginCapture(d->capture_stream, hipStreamCaptureModeThreadLocal));
             int rc = run_step(pipeline, d, d->capture_stream, nullptr, nullptr, nullptr);
             if (rc == SLOSH_OK) rc = run_step(pipeline, d, d->capture_stream, nullptr, nullptr, nullptr);
@@ -804,6 +880,7 @@ int slosh_launch_steps_graph(slosh_pipeline *pipeline, slosh_data *d, slosh_stre
             d->cur_p = save_p;
             d->cur_g = save_g;
             d->has_force = save_force;
+            d->next_primed = save_primed;
             if (rc != SLOSH_OK) {
                 if (graph) (void)hipGraphDestroy(graph);
                 return rc;
@@ -837,12 +914,13 @@ int slosh_set_sim_params(slosh_data *d, const slosh_sim_params *params) {
 int slosh_check_errors(slosh_data *d, slosh_stream stream) {
     if (!d) return fail(SLOSH_E_INVALID, "null data");
     hipStream_t st = (hipStream_t)stream;
-    uint32_t c[8];
-    HIP_TRY(hipMemcpyAsync(c, d->counters, 32, hipMemcpyDeviceToHost, st));
+    uint32_t c[16];
+    HIP_TRY(hipMemcpyAsync(c, d->counters, 64, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if (c[CTR_ERRORS] != 0u) {
+    if ((c[CTR_ERRORS] | c[8 + CTR_ERRORS]) != 0u) {
         uint32_t zero = 0;
         HIP_TRY(hipMemcpy(d->counters + CTR_ERRORS, &zero, 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d->counters + 8 + CTR_ERRORS, &zero, 4, hipMemcpyHostToDevice));
         return fail(SLOSH_E_GRID_OVERFLOW, "sparse-grid hash map overflow (increase grid_capacity)");
     }
     return SLOSH_OK;
@@ -953,7 +1031,7 @@ int slosh_num_blocks(slosh_data *d, slosh_stream stream, uint32_t *out) {
     if (!d || !out) return fail(SLOSH_E_INVALID, "null argument");
     hipStream_t st = (hipStream_t)stream;
     uint32_t c[8];
-    HIP_TRY(hipMemcpyAsync(c, d->counters, 32, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(c, ctr_set(d, d->cur_g), 32, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     *out = c[CTR_NUM_BLOCKS];
     return SLOSH_OK;
@@ -1154,6 +1232,8 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
         TRY(rebuild(d, st, (d->n + particles->len) * 3 / 2 + 1024, all, any_sand));
     }
     if (any_force) d->has_force = 1;
+    d->tag_mask = tag_mask_of(d->models_host);
+    update_uniform(d);
     return SLOSH_OK;
 }
 
@@ -1198,6 +1278,8 @@ static int remove_ids(slosh_data *d, hipStream_t st, const std::vector<uint8_t> 
     d->models_host = models;
     d->props_host = props;
     d->scene_host = scenes;
+    d->tag_mask = tag_mask_of(d->models_host);
+    update_uniform(d);
     TRY(rebuild(d, st, d->pcap, r, false));
     d->has_force = any_force ? 1u : 0u;
     return SLOSH_OK;
@@ -1280,7 +1362,7 @@ int slosh_device_view_get(slosh_data *d, slosh_device_view *out) {
     out->force_dt = d->fdt;
     out->nodes = d->nodes;
     out->sorted_ids = d->perm;
-    out->counters = d->counters;
+    out->counters = ctr_set(d, d->cur_g);
     return SLOSH_OK;
 }
 
