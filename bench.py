@@ -5,14 +5,18 @@ Default workload: the 1,048,576-particle dam break (weakly compressible
 neo-Hookean column, dx = 1/256 sparse hashed grid, dt = 2e-5, separating
 walls), seeded synthetic input resident in HBM before timing starts.
 
-Timed region: W warm-up substeps, then exactly K substeps replayed through a
-captured HIP graph, bracketed by stream synchronizations and HIP events on the
-launch stream. value = N * K / t.
+Timed region: W warm-up substeps, then exactly K substeps captured once into a
+HIP graph and replayed, bracketed by stream synchronizations and HIP events on
+the launch stream. value = N * K / t.
 
-Roofline: a second region of K substeps is launched eagerly with per-pass HIP
-events (the library's timestamps, on the launch stream); the pass with the
-largest time is the dominant kernel and its achieved HBM rate is its
-algorithmic bytes per launch divided by its mean launch time.
+Roofline: K more substeps launched directly, each pass timed by HIP events that
+its kernels' dispatch packets write (hipExtLaunchKernelGGL, on the launch
+stream); the pass with the largest time is the dominant kernel and its achieved
+HBM rate is its algorithmic bytes per launch divided by its mean launch time.
+
+roofline.traffic: HBM bytes per launch of the dominant kernel from the rocprofv3
+FETCH_SIZE / WRITE_SIZE summary committed under profiles/ for the default
+workload (null for other workloads or when the summary is absent).
 
 cpu_baseline: the CPU oracle (a C restatement of the reference substep, OpenMP)
 timed on the host cores on the first S substeps of the same scene.
@@ -48,6 +52,20 @@ def pass_bytes(n, n_blk, n_dp=0):
     }
 
 
+def measured_traffic(workload, pass_name):
+    """HBM bytes per launch of `pass_name`'s kernel from the committed PMC summary."""
+    if workload != "dam_break":
+        return None
+    path = os.path.join(ROOT, "profiles", "dam_break_1M_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        f = ln.split()
+        if len(f) == 4 and f[0] == "traffic" and f[1] == pass_name:
+            return float(f[3])
+    return None
+
+
 def make_scene(workload):
     from slosh_amd import scenes
     if workload == "dam_break":
@@ -80,13 +98,15 @@ def run_plain(args):
     st = P.Stream()
     data = P.MpmData.from_scene(pl, scene)
     # warm-up (also sorts particle storage by block)
-    pl.launch_steps_graph(data, args.warmup, st)
+    pl.launch_step(data, st)
+    pl.launch_steps_graph(data, max(args.warmup - 1, 0), st)
     st.synchronize()
     data.check_errors(st)
+    g_plain = P.Graph(pl, data, args.steps)
     e0, e1 = P.Event(), P.Event()
     st.synchronize()
     e0.record(st)
-    pl.launch_steps_graph(data, args.steps, st)
+    g_plain.launch(st)
     e1.record(st)
     st.synchronize()
     ms = e0.elapsed_ms(e1)
@@ -94,7 +114,7 @@ def run_plain(args):
     n_blk = data.num_blocks(st)
     value = n * args.steps / (ms * 1e-3)
 
-    # Per-pass timing region (eager launches with per-pass events).
+    # Per-pass times: K more substeps with packet-written events around every pass.
     ts = P.Timestamps()
     ts.reset()
     for _ in range(args.steps):
@@ -109,7 +129,7 @@ def run_plain(args):
             v["GBps"] = nbytes[k] / (v["mean_us"] * 1e-6) / 1e9
     dom = max((k for k in passes if k in nbytes), key=lambda k: passes[k]["mean_us"])
     achieved = passes[dom]["GBps"]
-    eager_ms = sum(v[0] for v in per_pass.values()) / args.steps
+    timed_ms = sum(v[0] for v in per_pass.values()) / args.steps
     line = {
         "metric": METRIC,
         "value": value,
@@ -129,7 +149,7 @@ def run_plain(args):
             "cell_width": float(scene.cell_width),
             "dt": float(scene.params["dt"]),
             "active_blocks": int(n_blk),
-            "launch": "hipGraph (2 substeps per graph)",
+            "launch": f"hipGraph of {args.steps} substeps",
         },
         "roofline": {
             "bound": "hbm",
@@ -138,10 +158,10 @@ def run_plain(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.workload, dom),
         },
         "passes": passes,
-        "eager_ms_per_step_sum_of_passes": eager_ms,
+        "sum_of_pass_ms_per_step": timed_ms,
         "substep_alg_bytes": sum(nbytes.values()),
         "substep_alg_GBps": sum(nbytes.values()) / (ms / args.steps * 1e-3) / 1e9,
     }
@@ -157,12 +177,13 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="dam_break", choices=["dam_break", "dam_break_small"])
-    ap.add_argument("--cpu-steps", type=int, default=40, help="oracle substeps for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=120, help="oracle substeps for cpu_baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
+    args.steps += args.steps & 1  # the captured graph holds an even number of substeps
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
-                                   "DESIGN.md and not implemented", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU only (--gpus must be 1); sharding across "
+                                   "GPUs is not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
     line = run_plain(args)
     print(json.dumps(line), flush=True)

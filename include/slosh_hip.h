@@ -215,6 +215,17 @@ int slosh_launch_step(slosh_pipeline *pipeline, slosh_data *data, slosh_stream s
 int slosh_launch_steps_graph(slosh_pipeline *pipeline, slosh_data *data, slosh_stream stream,
                              uint32_t count);
 
+/* A captured sequence of substeps (no reference counterpart: wgpu command buffers
+ * are recorded per frame; this is their hipGraph equivalent). `steps` must be even
+ * (the substep alternates particle buffers and hash tables, so the state after an
+ * even count matches the capture state); capture after at least one
+ * slosh_launch_step. A graph is invalidated by any mutation of `data` (launch then
+ * fails with SLOSH_E_INVALID). */
+typedef struct slosh_graph slosh_graph;
+int slosh_graph_capture(slosh_pipeline *pipeline, slosh_data *data, uint32_t steps, slosh_graph **out);
+int slosh_graph_launch(slosh_graph *graph, slosh_stream stream);
+void slosh_graph_destroy(slosh_graph *graph);
+
 /* Precision of the SVD-based steps (fixed-corotated stress, Drucker-Prager
  * return mapping): 1 = FP64 (default), 0 = FP32 as the reference computes them.
  * No reference counterpart: the reference is FP32 only. */
@@ -259,7 +270,9 @@ int slosh_device_view_get(slosh_data *data, slosh_device_view *out);
 
 /* ---- Timing: per-pass GPU timestamps (GpuTimestamps; pass labels of
  * src/pipeline.rs:588-757: grid_sort, p2g, grid_update, g2p, particles_update,
- * integrate_bodies). The fused G2P + particle-update kernel is reported as g2p. */
+ * integrate_bodies). The timestamps are written by the dispatch packets of each
+ * pass's first and last kernel. The fused G2P + particle-update kernel (which also
+ * integrates the rigid bodies) is reported as g2p. */
 #define SLOSH_NUM_PASSES 6
 int slosh_timestamps_create(slosh_timestamps **out);
 void slosh_timestamps_destroy(slosh_timestamps *ts);

@@ -20,7 +20,8 @@ EXPORTS = [
     "slosh_device_view_get", "slosh_timestamps_create", "slosh_timestamps_destroy", "slosh_timestamps_read",
     "slosh_timestamps_reset", "slosh_stream_create", "slosh_stream_destroy", "slosh_stream_synchronize",
     "slosh_event_create", "slosh_event_destroy", "slosh_event_record", "slosh_event_elapsed_ms",
-    "slosh_device_count", "slosh_last_error", "slosh_version",
+    "slosh_device_count", "slosh_last_error", "slosh_version", "slosh_graph_capture", "slosh_graph_launch",
+    "slosh_graph_destroy",
 ]
 
 
@@ -111,6 +112,9 @@ def load():
         "slosh_event_record": ([vp, vp], i32),
         "slosh_event_elapsed_ms": ([vp, vp, vp], i32),
         "slosh_device_count": ([vp], i32),
+        "slosh_graph_capture": ([vp, vp, u32, vp], i32),
+        "slosh_graph_launch": ([vp, vp], i32),
+        "slosh_graph_destroy": ([vp], None),
         "slosh_last_error": ([], C.c_char_p),
         "slosh_version": ([], C.c_char_p),
     }

@@ -6,6 +6,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 
 #define SLOSH_NONE 0xffffffffu
@@ -119,6 +120,23 @@ __device__ inline uint32_t hash_find(const uint32_t *__restrict__ keys, const ui
         slot = (slot + 1u) & (cap - 1u);
     }
     return SLOSH_NONE;
+}
+
+// Optional per-pass timing: the dispatch packets of the pass's first / last
+// kernel write the start / stop timestamps (hipExtLaunchKernelGGL), so the
+// measured time is the kernels' own even when the host enqueues slower than the
+// GPU executes.
+struct PassEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+template <typename... KArgs, typename... Args>
+inline void launch_kernel(const PassEvents *ev, void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32_t shmem,
+                          hipStream_t stream, Args... args) {
+    if (ev && (ev->start || ev->stop))
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, ev->start, ev->stop, 0u, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
 }
 
 // Wave-level helpers (64-lane wavefronts).

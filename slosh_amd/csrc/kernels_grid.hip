@@ -241,27 +241,43 @@ static inline uint32_t div_up(uint64_t a, uint32_t b) { return (uint32_t)((a + b
 static inline uint32_t min_u(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
 // `primed`: the previous substep already reset this table and inserted every
-// particle's primary block (fused into its P2G and G2P kernels).
+// particle's primary block (fused into its P2G and G2P kernels). `ev` (optional)
+// brackets the whole sequence.
 hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n, float h, uint32_t batched,
-                             uint32_t cap, bool primed, hipStream_t stream) {
+                             uint32_t cap, bool primed, const PassEvents *ev, hipStream_t stream) {
     const uint32_t slot_grid = min_u(div_up(cap, 256), 2048);
     const uint32_t part_grid = div_up(n, 256);
-    if (!primed) {
-        hipLaunchKernelGGL(k_reset_table, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, g.counters);
-        if (n > 0) hipLaunchKernelGGL(k_touch_primary, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched,
-                                      g.keys, g.vals, cap, g.counters);
+    PassEvents first, last;
+    if (ev) {
+        first.start = ev->start;
+        last.stop = ev->stop;
     }
-    hipLaunchKernelGGL(k_touch_neighbors, dim3(min_u(div_up((uint64_t)cap * 8u, 256), 8192)), dim3(256), 0, stream,
-                       g.keys, g.vals, cap, batched, g.counters);
-    hipLaunchKernelGGL(k_compact, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, g.hdr_key, g.cell_cnt,
-                       g.counters);
-    if (n > 0) hipLaunchKernelGGL(k_count, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched, g.keys, g.vals,
-                                  cap, g.cell_cnt, g.pslot, g.prank, g.counters);
+    const PassEvents *f = ev ? &first : nullptr;
+    if (!primed) {
+        launch_kernel(f, k_reset_table, dim3(slot_grid), dim3(256), 0, stream, g.keys, g.vals, cap, g.counters);
+        f = nullptr;
+        if (n > 0)
+            launch_kernel(f, k_touch_primary, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched, g.keys,
+                          g.vals, cap, g.counters);
+    }
+    launch_kernel(f, k_touch_neighbors, dim3(min_u(div_up((uint64_t)cap * 8u, 256), 8192)), dim3(256), 0, stream,
+                  g.keys, g.vals, cap, batched, g.counters);
+    launch_kernel(nullptr, k_compact, dim3(slot_grid), dim3(256), 0, stream, (const uint32_t *)g.keys, g.vals, cap,
+                  g.hdr_key, g.cell_cnt, g.counters);
+    if (n > 0)
+        launch_kernel(nullptr, k_count, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched,
+                      (const uint32_t *)g.keys, (const uint32_t *)g.vals, cap, g.cell_cnt, g.pslot, g.prank,
+                      g.counters);
     const uint32_t blk_grid = min_u(div_up(cap, 4), 4096);
-    hipLaunchKernelGGL(k_block_prepare, dim3(blk_grid), dim3(256), 0, stream, g.cell_cnt, g.cell_start, g.blk_cnt,
-                       g.hdr_key, g.nbr, g.keys, g.vals, g.prev_keys, g.prev_vals, cap, batched, g.counters);
-    hipLaunchKernelGGL(k_block_scan, dim3(1), dim3(1024), 0, stream, g.blk_cnt, g.blk_start, g.counters);
-    if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(part_grid), dim3(256), 0, stream, g.pslot, g.prank, g.cell_start,
-                                  g.blk_start, g.perm, n);
+    launch_kernel(nullptr, k_block_prepare, dim3(blk_grid), dim3(256), 0, stream, (const uint32_t *)g.cell_cnt,
+                  g.cell_start, g.blk_cnt, (const uint32_t *)g.hdr_key, g.nbr, (const uint32_t *)g.keys,
+                  (const uint32_t *)g.vals, (const uint32_t *)g.prev_keys, (const uint32_t *)g.prev_vals, cap, batched,
+                  (const uint32_t *)g.counters);
+    launch_kernel(n > 0 ? nullptr : (ev ? &last : nullptr), k_block_scan, dim3(1), dim3(1024), 0, stream,
+                  (const uint32_t *)g.blk_cnt, g.blk_start, g.counters);
+    if (n > 0)
+        launch_kernel(ev ? &last : nullptr, k_scatter, dim3(part_grid), dim3(256), 0, stream,
+                      (const uint32_t *)g.pslot, (const uint32_t *)g.prank, (const uint32_t *)g.cell_start,
+                      (const uint32_t *)g.blk_start, g.perm, n);
     return hipGetLastError();
 }

@@ -1146,34 +1146,35 @@ static inline PBuf to_pbuf(const ParticleBuffers &b) {
 uint32_t block_grid_size(uint32_t cap, uint32_t per_wg) { return min_t(div_up_t(cap, per_wg), 8192); }
 
 hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t pcap, const TransferBuffers &t,
-                      const NextTableBuffers &next, hipStream_t stream) {
+                      const NextTableBuffers &next, const PassEvents *ev, hipStream_t stream) {
     NextTable nt;
     nt.keys = next.keys;
     nt.vals = next.vals;
     nt.counters = next.counters;
     nt.prime = next.prime;
-    hipLaunchKernelGGL(k_p2g<512>, dim3(block_grid_size(P.cap, 1)), dim3(192), 0, stream, P, nt, pb.pos, pb.vm, pb.C,
-                       t.fdt, pcap, t.perm, t.cell_cnt, t.cell_start, t.blk_start, t.blk_cnt, t.hdr_key, t.tiles,
-                       t.counters);
+    launch_kernel(ev, k_p2g<512>, dim3(block_grid_size(P.cap, 1)), dim3(192), 0, stream, P, nt,
+                  (const float4 *)pb.pos, (const float4 *)pb.vm, (const float *)pb.C, (const float *)t.fdt, pcap,
+                  t.perm, t.cell_cnt, t.cell_start, t.blk_start, t.blk_cnt, t.hdr_key, t.tiles, t.counters);
     return hipGetLastError();
 }
 
 hipError_t launch_grid_update(const StepParams &P, const TransferBuffers &t, const ColliderBuffers &c, uint32_t mode,
-                              uint32_t do_update, hipStream_t stream) {
+                              uint32_t do_update, const PassEvents *ev, hipStream_t stream) {
     Colliders cs;
     cs.shapes = c.shapes;
     cs.poses = c.poses;
     cs.mats = c.mats;
     cs.count = c.count;
-    hipLaunchKernelGGL(k_grid_update, dim3(block_grid_size(P.cap, 4)), dim3(256), 0, stream, P, t.tiles, t.nodes, t.nbr,
-                       t.hdr_key, cs, (const NodeColl *)t.coll_prev, (NodeColl *)t.coll_cur, t.counters, mode,
-                       do_update);
+    launch_kernel(ev, k_grid_update, dim3(block_grid_size(P.cap, 4)), dim3(256), 0, stream, P,
+                  (const float4 *)t.tiles, t.nodes, t.nbr, t.hdr_key, cs, (const NodeColl *)t.coll_prev,
+                  (NodeColl *)t.coll_cur, t.counters, mode, do_update);
     return hipGetLastError();
 }
 
 hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, const ParticleBuffers &out,
                              const StaticBuffers &s, uint32_t pcap, const TransferBuffers &t,
-                             const NextTableBuffers &next, const BodyBuffers &bb, hipStream_t stream) {
+                             const NextTableBuffers &next, const BodyBuffers &bb, const PassEvents *ev,
+                             hipStream_t stream) {
     NextTable nt;
     nt.keys = next.keys;
     nt.vals = next.vals;
@@ -1192,25 +1193,25 @@ hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, con
     st.dp0 = s.dp0;
     st.dp1 = s.dp1;
     if (P.tag_mask == 2u)  // neo-Hookean only
-        hipLaunchKernelGGL(k_g2p_update<2u>, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(in),
-                           to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, t.nodes, t.counters, nt,
-                           b);
+        launch_kernel(ev, k_g2p_update<2u>, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(in),
+                      to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes,
+                      t.counters, nt, b);
     else
-        hipLaunchKernelGGL(k_g2p_update<15u>, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(in),
-                           to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, t.nodes, t.counters, nt,
-                           b);
+        launch_kernel(ev, k_g2p_update<15u>, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(in),
+                      to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes,
+                      t.counters, nt, b);
     return hipGetLastError();
 }
 
 hipError_t launch_g2p(const StepParams &P, const ParticleBuffers &b, uint32_t pcap, const TransferBuffers &t,
-                      hipStream_t stream) {
-    hipLaunchKernelGGL(k_g2p, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(b), pcap, t.perm,
-                       t.blk_start, t.blk_cnt, t.nbr, t.nodes, t.counters);
+                      const PassEvents *ev, hipStream_t stream) {
+    launch_kernel(ev, k_g2p, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(b), pcap, t.perm,
+                  t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes, t.counters);
     return hipGetLastError();
 }
 
 hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
-                                  float *fdt, hipStream_t stream) {
+                                  float *fdt, const PassEvents *ev, hipStream_t stream) {
     if (P.n == 0) return hipSuccess;
     PStatic st;
     st.prop = s.prop;
@@ -1218,15 +1219,15 @@ hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b,
     st.dp0 = s.dp0;
     st.dp1 = s.dp1;
     if (P.tag_mask == 2u)
-        hipLaunchKernelGGL(k_particle_update<2u>, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st,
-                           pcap, fdt);
+        launch_kernel(ev, k_particle_update<2u>, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st,
+                      pcap, fdt);
     else
-        hipLaunchKernelGGL(k_particle_update<15u>, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st,
-                           pcap, fdt);
+        launch_kernel(ev, k_particle_update<15u>, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st,
+                      pcap, fdt);
     return hipGetLastError();
 }
 
-hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, hipStream_t stream) {
+hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, const PassEvents *ev, hipStream_t stream) {
     if (bb.count == 0) return hipSuccess;
     Bodies b;
     b.poses = bb.poses;
@@ -1235,6 +1236,6 @@ hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, hipSt
     b.local_com = bb.local_com;
     b.impulses = bb.impulses;
     b.count = bb.count;
-    hipLaunchKernelGGL(k_rigid_update, dim3(1), dim3(64), 0, stream, P, b);
+    launch_kernel(ev, k_rigid_update, dim3(1), dim3(64), 0, stream, P, b);
     return hipGetLastError();
 }

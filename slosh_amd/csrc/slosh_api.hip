@@ -38,7 +38,7 @@ struct GridBuffers {
     uint32_t *pslot, *prank, *perm, *counters;
 };
 hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n, float h, uint32_t batched,
-                             uint32_t cap, bool primed, hipStream_t stream);
+                             uint32_t cap, bool primed, const PassEvents *ev, hipStream_t stream);
 struct TransferBuffers {
     const uint32_t *perm, *cell_cnt, *cell_start, *blk_start, *blk_cnt, *hdr_key, *nbr, *counters;
     float4 *tiles, *nodes;
@@ -69,17 +69,18 @@ struct NextTableBuffers {
     uint32_t prime;
 };
 hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t pcap, const TransferBuffers &t,
-                      const NextTableBuffers &next, hipStream_t stream);
+                      const NextTableBuffers &next, const PassEvents *ev, hipStream_t stream);
 hipError_t launch_grid_update(const StepParams &P, const TransferBuffers &t, const ColliderBuffers &c, uint32_t mode,
-                              uint32_t do_update, hipStream_t stream);
+                              uint32_t do_update, const PassEvents *ev, hipStream_t stream);
 hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, const ParticleBuffers &out,
                              const StaticBuffers &s, uint32_t pcap, const TransferBuffers &t,
-                             const NextTableBuffers &next, const BodyBuffers &bb, hipStream_t stream);
+                             const NextTableBuffers &next, const BodyBuffers &bb, const PassEvents *ev,
+                             hipStream_t stream);
 hipError_t launch_g2p(const StepParams &P, const ParticleBuffers &b, uint32_t pcap, const TransferBuffers &t,
-                      hipStream_t stream);
+                      const PassEvents *ev, hipStream_t stream);
 hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
-                                  float *fdt, hipStream_t stream);
-hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, hipStream_t stream);
+                                  float *fdt, const PassEvents *ev, hipStream_t stream);
+hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, const PassEvents *ev, hipStream_t stream);
 
 // ---- error plumbing ---------------------------------------------------------
 static thread_local std::string g_last_error;
@@ -168,6 +169,7 @@ struct slosh_data {
     int *impulses = nullptr;
     // captured substep pairs, one per starting parity
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t generation = 0;  // bumped by every mutation; user graphs check it
     hipStream_t capture_stream = nullptr;
 };
 
@@ -207,6 +209,7 @@ static uint32_t *ctr_set(slosh_data *d, int g) { return d->counters + 8 * g; }
 
 static void invalidate_graphs(slosh_data *d) {
     d->next_primed = false;
+    d->generation++;
     for (int i = 0; i < 4; i++)
         if (d->graph[i]) {
             (void)hipGraphExecDestroy(d->graph[i]);
@@ -483,25 +486,24 @@ static StaticBuffers sbufs(slosh_data *d) {
     return s;
 }
 
-static void record(slosh_timestamps *ts, int pass, hipStream_t st, bool begin) {
-    if (!ts) return;
-    if (begin) {
-        slosh_timestamps::Rec r;
-        r.pass = pass;
-        if (ts->pool.size() >= 2) {
-            r.a = ts->pool.back();
-            ts->pool.pop_back();
-            r.b = ts->pool.back();
-            ts->pool.pop_back();
-        } else {
-            (void)hipEventCreate(&r.a);
-            (void)hipEventCreate(&r.b);
-        }
-        (void)hipEventRecord(r.a, st);
-        ts->recs.push_back(r);
+// Events bracketing one pass (written by the dispatch packets), or nullptr.
+static const PassEvents *pass_events(slosh_timestamps *ts, int pass, PassEvents *storage) {
+    if (!ts) return nullptr;
+    slosh_timestamps::Rec r;
+    r.pass = pass;
+    if (ts->pool.size() >= 2) {
+        r.a = ts->pool.back();
+        ts->pool.pop_back();
+        r.b = ts->pool.back();
+        ts->pool.pop_back();
     } else {
-        (void)hipEventRecord(ts->recs.back().b, st);
+        (void)hipEventCreateWithFlags(&r.a, hipEventDisableSystemFence);
+        (void)hipEventCreateWithFlags(&r.b, hipEventDisableSystemFence);
     }
+    ts->recs.push_back(r);
+    storage->start = r.a;
+    storage->stop = r.b;
+    return storage;
 }
 
 // ---- one substep --------------------------------------------------------------
@@ -549,10 +551,10 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     next.counters = ctr_set(d, prev_g);
     next.prime = prime ? 1u : 0u;
 
-    record(ts, PASS_GRID_SORT, st, true);
-    HIP_TRY(launch_grid_build(g, d->pb[d->cur_p].pos, n, d->h, d->batched, d->cap, d->next_primed, st));
+    PassEvents pe;
+    HIP_TRY(launch_grid_build(g, d->pb[d->cur_p].pos, n, d->h, d->batched, d->cap, d->next_primed,
+                              pass_events(ts, PASS_GRID_SORT, &pe), st));
     d->next_primed = false;
-    record(ts, PASS_GRID_SORT, st, false);
     if (hooks && hooks->after_particle_sort && hooks->after_particle_sort(d, st, hs) != 0)
         return fail(SLOSH_E_HOOK, "after_particle_sort hook failed");
 
@@ -578,9 +580,7 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     if (!p2g_by_hook) {
         if (!K.builtin_transfers)
             return fail(SLOSH_E_NO_KERNELS, "pipeline built without builtin transfer kernels; a hook must handle run_p2g");
-        record(ts, PASS_P2G, st, true);
-        HIP_TRY(launch_p2g(P, pbufs(d, d->cur_p), (uint32_t)d->pcap, t, next, st));
-        record(ts, PASS_P2G, st, false);
+        HIP_TRY(launch_p2g(P, pbufs(d, d->cur_p), (uint32_t)d->pcap, t, next, pass_events(ts, PASS_P2G, &pe), st));
     }
     if (hooks && hooks->after_p2g && hooks->after_p2g(d, st, hs) != 0) return fail(SLOSH_E_HOOK, "after_p2g hook failed");
 
@@ -591,9 +591,8 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     cb.count = d->nbodies;
     if (K.grid_update || !p2g_by_hook) {
         const uint32_t mode = p2g_by_hook ? 1u : 0u;
-        record(ts, PASS_GRID_UPDATE, st, true);
-        HIP_TRY(launch_grid_update(P, t, cb, mode, K.grid_update ? 1u : 0u, st));
-        record(ts, PASS_GRID_UPDATE, st, false);
+        HIP_TRY(launch_grid_update(P, t, cb, mode, K.grid_update ? 1u : 0u, pass_events(ts, PASS_GRID_UPDATE, &pe),
+                                   st));
     }
     if (d->coll_cache) d->coll_prev_valid = 1;
     if (hooks && hooks->after_grid_update && hooks->after_grid_update(d, st, hs) != 0)
@@ -618,24 +617,19 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
         // Fused G2P + particle update; writes the next particle buffer in sorted order.
         // Fused G2P + particle update; writes the next particle buffer in sorted
         // order and (fused) integrates the rigid bodies.
-        record(ts, PASS_G2P, st, true);
         HIP_TRY(launch_g2p_update(P, pbufs(d, d->cur_p), pbufs(d, d->cur_p ^ 1), sbufs(d), (uint32_t)d->pcap, t, next,
-                                  bb, st));
-        record(ts, PASS_G2P, st, false);
+                                  bb, pass_events(ts, PASS_G2P, &pe), st));
         d->cur_p ^= 1;
         d->next_primed = next.prime != 0u;
     } else {
         if (!g2p_by_hook) {
-            record(ts, PASS_G2P, st, true);
-            HIP_TRY(launch_g2p(P, pbufs(d, d->cur_p), (uint32_t)d->pcap, t, st));
-            record(ts, PASS_G2P, st, false);
+            HIP_TRY(launch_g2p(P, pbufs(d, d->cur_p), (uint32_t)d->pcap, t, pass_events(ts, PASS_G2P, &pe), st));
         }
         if (hooks && hooks->after_g2p && hooks->after_g2p(d, st, hs) != 0)
             return fail(SLOSH_E_HOOK, "after_g2p hook failed");
         if (K.particles_update) {
-            record(ts, PASS_PARTICLES_UPDATE, st, true);
-            HIP_TRY(launch_particle_update(P, pbufs(d, d->cur_p), sbufs(d), (uint32_t)d->pcap, d->fdt, st));
-            record(ts, PASS_PARTICLES_UPDATE, st, false);
+            HIP_TRY(launch_particle_update(P, pbufs(d, d->cur_p), sbufs(d), (uint32_t)d->pcap, d->fdt,
+                                           pass_events(ts, PASS_PARTICLES_UPDATE, &pe), st));
         }
     }
     if (K.particles_update) d->has_force = 0;  // force_dt is zeroed by every particle update
@@ -643,9 +637,7 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
         return fail(SLOSH_E_HOOK, "after_particles_update hook failed");
 
     if (split && d->nbodies > 0) {
-        record(ts, PASS_INTEGRATE_BODIES, st, true);
-        HIP_TRY(launch_rigid_update(P, bb, st));
-        record(ts, PASS_INTEGRATE_BODIES, st, false);
+        HIP_TRY(launch_rigid_update(P, bb, pass_events(ts, PASS_INTEGRATE_BODIES, &pe), st));
     }
     return SLOSH_OK;
 }
@@ -895,6 +887,72 @@ int slosh_launch_steps_graph(slosh_pipeline *pipeline, slosh_data *d, slosh_stre
     }
     if (count == 1) TRY(run_step(pipeline, d, st, nullptr, nullptr, nullptr));
     return SLOSH_OK;
+}
+
+struct slosh_graph {
+    slosh_data *data = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t generation = 0;
+    int cur_p = 0, cur_g = 0;
+};
+
+int slosh_graph_capture(slosh_pipeline *pipeline, slosh_data *d, uint32_t steps, slosh_graph **out) {
+    if (!pipeline || !d || !out) return fail(SLOSH_E_INVALID, "null argument");
+    if (steps == 0 || (steps & 1u)) return fail(SLOSH_E_INVALID, "steps must be even and non-zero");
+    if (d->has_force || (d->coll_cache && !d->coll_prev_valid) || !d->next_primed)
+        return fail(SLOSH_E_INVALID, "run one slosh_launch_step before capturing (state not steady)");
+    *out = nullptr;
+    slosh_graph *g = new slosh_graph();
+    const int save_p = d->cur_p, save_g = d->cur_g;
+    const uint32_t save_force = d->has_force;
+    const bool save_primed = d->next_primed;
+    hipError_t e = hipStreamBeginCapture(d->capture_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(SLOSH_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    }
+    int rc = SLOSH_OK;
+    for (uint32_t i = 0; i < steps && rc == SLOSH_OK; i++)
+        rc = run_step(pipeline, d, d->capture_stream, nullptr, nullptr, nullptr);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(d->capture_stream, &graph);
+    const bool back = d->cur_p == save_p && d->cur_g == save_g && d->next_primed == save_primed;
+    d->cur_p = save_p;
+    d->cur_g = save_g;
+    d->has_force = save_force;
+    d->next_primed = save_primed;
+    if (rc == SLOSH_OK && e != hipSuccess) rc = fail(SLOSH_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if (rc == SLOSH_OK && !back) rc = fail(SLOSH_E_INVALID, "captured substeps do not return to the captured state");
+    if (rc == SLOSH_OK) {
+        e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = fail(SLOSH_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != SLOSH_OK) {
+        slosh_graph_destroy(g);
+        return rc;
+    }
+    g->data = d;
+    g->generation = d->generation;
+    g->cur_p = save_p;
+    g->cur_g = save_g;
+    *out = g;
+    return SLOSH_OK;
+}
+
+int slosh_graph_launch(slosh_graph *g, slosh_stream stream) {
+    if (!g || !g->exec) return fail(SLOSH_E_INVALID, "null graph");
+    slosh_data *d = g->data;
+    if (d->generation != g->generation || d->cur_p != g->cur_p || d->cur_g != g->cur_g || !d->next_primed)
+        return fail(SLOSH_E_INVALID, "graph no longer matches the data state (mutated or stepped an odd count)");
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return SLOSH_OK;
+}
+
+void slosh_graph_destroy(slosh_graph *g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    delete g;
 }
 
 int slosh_set_svd_fp64(slosh_data *d, int enable) {

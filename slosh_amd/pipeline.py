@@ -155,6 +155,25 @@ class MpmPipeline:
             pass
 
 
+class Graph:
+    """`steps` substeps of `data` captured once into a HIP graph and replayed with ``launch``."""
+
+    def __init__(self, pipeline, data, steps):
+        h = C.c_void_p()
+        _lib.check(_lib.load().slosh_graph_capture(pipeline.handle, data.handle, int(steps), C.byref(h)))
+        self.handle = h
+        self.steps = steps
+
+    def launch(self, stream=None):
+        _lib.check(_lib.load().slosh_graph_launch(self.handle, stream.handle if stream else None))
+
+    def __del__(self):
+        try:
+            _lib.load().slosh_graph_destroy(self.handle)
+        except Exception:
+            pass
+
+
 class Timestamps:
     PASSES = ["grid_sort", "p2g", "grid_update", "g2p", "particles_update", "integrate_bodies"]
 
