@@ -171,6 +171,96 @@ def run_plain(args):
     return line
 
 
+def run_batched(args):
+    """256 independent scenes in one batch vs the plain per-scene loop (each scene
+    its own MpmData stepped by its own captured graphs, one after another)."""
+    from slosh_amd import pipeline as P
+    from slosh_amd import scenes
+    scs = [scenes.batched_scene(i) for i in range(args.batch_scenes)]
+    batch = scenes.Batch(scs)
+    pl = P.MpmPipeline()
+    st = P.Stream()
+    data = P.MpmData.from_batch(pl, batch)
+    pl.launch_step(data, st)
+    pl.launch_steps_graph(data, max(args.warmup - 1, 0), st)
+    st.synchronize()
+    data.check_errors(st)
+    g = P.Graph(pl, data, args.batch_steps)
+    e0, e1 = P.Event(), P.Event()
+    st.synchronize()
+    e0.record(st)
+    g.launch(st)
+    e1.record(st)
+    st.synchronize()
+    ms = e0.elapsed_ms(e1)
+    data.check_errors(st)
+    n_blk = data.num_blocks(st)
+    batched = batch.n * args.batch_steps / (ms * 1e-3)
+    g = None
+    data.close()
+    # Plain per-scene loop over the same scenes and substeps.
+    loop_ms = 0.0
+    for sc in scs:
+        cap = scenes.pow2_at_least(scenes.estimate_blocks(sc.particles["pos4"], sc.cell_width) * 2 + 256)
+        d = P.MpmData(pl, sc.params, sc.particles, sc.bodies, sc.materials, sc.cell_width, cap)
+        pl.launch_step(d, st)
+        pl.launch_steps_graph(d, max(args.warmup - 1, 0), st)
+        gs = P.Graph(pl, d, args.batch_steps)
+        st.synchronize()
+        e0.record(st)
+        gs.launch(st)
+        e1.record(st)
+        st.synchronize()
+        loop_ms += e0.elapsed_ms(e1)
+        d.check_errors(st)
+        gs = None
+        d.close()
+    loop = batch.n * args.batch_steps / (loop_ms * 1e-3)
+    return {
+        "workload": f"{len(scs)} scenes, {batch.n} particles (N log-uniform [2e3, 2e5]; NH / FC / DP)",
+        "value": batched,
+        "unit": "particle-substeps/s",
+        "steps": args.batch_steps,
+        "ms_per_step": ms / args.batch_steps,
+        "active_blocks": int(n_blk),
+        "per_scene_loop": loop,
+        "per_scene_loop_ms_per_step": loop_ms / args.batch_steps,
+        "speedup_vs_per_scene_loop": batched / loop,
+    }
+
+
+def run_served(args):
+    """Poisson arrivals of scene requests served by continuous batching, against
+    the same requests served one at a time."""
+    from slosh_amd import pipeline as P
+    from slosh_amd import scenes
+    from slosh_amd import serving as SV
+    pl = P.MpmPipeline()
+    st = P.Stream()
+    out = []
+    gen = [scenes.served_request(k) for k in range(args.serve_requests)]
+    for rate in args.serve_rates:
+        arr = scenes.poisson_arrivals(rate, args.serve_requests)
+        reqs = [SV.Request(k, sc, steps, arrival=float(a)) for k, ((sc, steps), a) in enumerate(zip(gen, arr))]
+        srv = SV.SceneServer(pl, reqs[0].scene, hbm_budget=args.hbm_budget, stream=st)
+        done = srv.serve(reqs, realtime=True)
+        summ = SV.latency_summary(done)
+        summ.update({"rate_per_s": rate, "peak_particles_in_flight": srv.peak_in_flight,
+                     "device_bytes": srv.device_bytes, "chunks": srv.chunks})
+        srv.data.close()
+        out.append(summ)
+    # The plain loop at the highest rate (the most overlap to gain).
+    rate = max(args.serve_rates)
+    arr = scenes.poisson_arrivals(rate, args.serve_requests)
+    reqs = [SV.Request(k, sc, steps, arrival=float(a)) for k, ((sc, steps), a) in enumerate(zip(gen, arr))]
+    SV.serve_one_at_a_time(pl, reqs, stream=st, realtime=True)
+    plain = SV.latency_summary(reqs)
+    plain["rate_per_s"] = rate
+    best = [o for o in out if o["rate_per_s"] == rate][0]
+    return {"requests": args.serve_requests, "hbm_budget_bytes": args.hbm_budget, "rates": out,
+            "one_at_a_time": plain, "speedup_vs_one_at_a_time": best["throughput"] / plain["throughput"]}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +269,11 @@ def main():
     ap.add_argument("--workload", default="dam_break", choices=["dam_break", "dam_break_small"])
     ap.add_argument("--cpu-steps", type=int, default=120, help="oracle substeps for cpu_baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched leg (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--serve-requests", type=int, default=32, help="requests per served run (0 = skip)")
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[10.0, 50.0, 200.0])
+    ap.add_argument("--hbm-budget", type=float, default=200e9)
     args = ap.parse_args()
     args.steps += args.steps & 1  # the captured graph holds an even number of substeps
     if args.gpus != 1:
@@ -186,6 +281,11 @@ def main():
                                    "GPUs is not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
     line = run_plain(args)
+    if args.batch_scenes > 0:
+        args.batch_steps += args.batch_steps & 1
+        line["batched"] = run_batched(args)
+    if args.serve_requests > 0 and args.serve_rates:
+        line["served"] = run_served(args)
     print(json.dumps(line), flush=True)
 
 

@@ -258,11 +258,24 @@ int slosh_read_nodes(slosh_data *data, slosh_stream stream, float *out4, uint32_
 /* Scene ids of the blocks returned by slosh_read_blocks (batched mode; 0 otherwise). */
 int slosh_read_block_scenes(slosh_data *data, slosh_stream stream, uint32_t *out, uint32_t max_blocks);
 
-/* ---- Mutation: GpuParticles::append / shift_remove (src/solver/particle.rs:505-572). */
+/* ---- Mutation: GpuParticles::append / shift_remove (src/solver/particle.rs:505-572).
+ * Appending within the capacity given at creation uploads only the new particles;
+ * beyond it the storage is reallocated. Removal runs on the device (order-preserving
+ * compaction; the remaining particles keep their relative creation order and are
+ * renumbered from 0). Both are enqueued on `stream` and invalidate captured graphs. */
 int slosh_append_particles(slosh_data *data, slosh_stream stream, const slosh_particles *particles);
 int slosh_remove_particles(slosh_data *data, slosh_stream stream, size_t start, size_t end);
-/* Removes every particle of `scene_id` (batched mode); keeps the others' relative order. */
+/* Removes every particle of `scene_id` / of the listed scenes (batched mode; no
+ * reference counterpart: scenes are this library's batching unit). */
 int slosh_remove_scene(slosh_data *data, slosh_stream stream, uint32_t scene_id, size_t *removed);
+int slosh_remove_scenes(slosh_data *data, slosh_stream stream, const uint32_t *scene_ids, size_t count,
+                        size_t *removed);
+/* Number of particles of a scene, and their state in creation order (positions as
+ * float4 with w = 0; kinematics / def_grad in reference layouts; any output may be
+ * null). Gathered on the device; synchronous on `stream`. */
+size_t slosh_scene_size(const slosh_data *data, uint32_t scene_id);
+int slosh_read_scene(slosh_data *data, slosh_stream stream, uint32_t scene_id, float *positions4,
+                     slosh_kinematics *kinematics, slosh_matrix *def_grad, size_t max_count, size_t *count);
 /* Overwrites kinematics (e.g. force_dt) in the original particle order. */
 int slosh_write_kinematics(slosh_data *data, slosh_stream stream, const slosh_kinematics *kin);
 

@@ -16,7 +16,8 @@ EXPORTS = [
     "slosh_check_errors", "slosh_num_particles", "slosh_read_positions", "slosh_read_kinematics",
     "slosh_read_def_grad", "slosh_read_models", "slosh_read_properties", "slosh_read_scene_ids",
     "slosh_read_bodies", "slosh_num_blocks", "slosh_read_blocks", "slosh_read_nodes", "slosh_read_block_scenes",
-    "slosh_append_particles", "slosh_remove_particles", "slosh_remove_scene", "slosh_write_kinematics",
+    "slosh_append_particles", "slosh_remove_particles", "slosh_remove_scene", "slosh_remove_scenes",
+    "slosh_scene_size", "slosh_read_scene", "slosh_write_kinematics",
     "slosh_device_view_get", "slosh_timestamps_create", "slosh_timestamps_destroy", "slosh_timestamps_read",
     "slosh_timestamps_reset", "slosh_stream_create", "slosh_stream_destroy", "slosh_stream_synchronize",
     "slosh_event_create", "slosh_event_destroy", "slosh_event_record", "slosh_event_elapsed_ms",
@@ -98,6 +99,9 @@ def load():
         "slosh_append_particles": ([vp, vp, vp], i32),
         "slosh_remove_particles": ([vp, vp, sz, sz], i32),
         "slosh_remove_scene": ([vp, vp, u32, vp], i32),
+        "slosh_remove_scenes": ([vp, vp, vp, sz, vp], i32),
+        "slosh_scene_size": ([vp, u32], sz),
+        "slosh_read_scene": ([vp, vp, u32, vp, vp, vp, sz, vp], i32),
         "slosh_write_kinematics": ([vp, vp, vp], i32),
         "slosh_device_view_get": ([vp, vp], i32),
         "slosh_timestamps_create": ([vp], i32),

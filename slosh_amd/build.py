@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_PATH = os.path.join(HERE, "libslosh_hip.so")
-SOURCES = ["kernels_grid.hip", "kernels_transfer.hip", "slosh_api.hip"]
+SOURCES = ["kernels_grid.hip", "kernels_transfer.hip", "kernels_particles.hip", "slosh_api.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -53,10 +53,12 @@ __device__ inline void k_eval_all(float x, float *w) {
 // 3x3 helpers on shader matrices stored row-major in float[9].
 
 __device__ inline float det3(const float *a) {
+#pragma clang fp contract(off)
     return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
 }
 
 __device__ inline void mul3(const float *a, const float *b, float *r) {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -65,6 +67,7 @@ __device__ inline void mul3(const float *a, const float *b, float *r) {
 
 // mul(transpose(a), b)
 __device__ inline void mul3_tn(const float *a, const float *b, float *r) {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -190,6 +193,7 @@ __device__ inline void svd_recompose(const T *u, const T *s, const T *v, T *r) {
 
 // neo_hookean_elasticity.slang:16-27
 __device__ inline void nh_stress(float lambda, float mu, const float *f, float *st) {
+#pragma clang fp contract(off)
     float j = fmaxf(det3(f), 1.0e-10f);
     float diag = lambda * logf(j) - mu;
     mul3_tn(f, f, st);
@@ -202,6 +206,7 @@ __device__ inline void nh_stress(float lambda, float mu, const float *f, float *
 
 // linear_elasticity.slang:135-148 (fixed-corotated)
 __device__ inline void fc_stress(float lambda, float mu, const float *f, float *st) {
+#pragma clang fp contract(off)
     float u[9], s[3], v[9], rc[9];
     svd3<float>(f, u, s, v);
     float j = s[0] * s[1] * s[2];
@@ -526,13 +531,17 @@ __device__ inline void g2p_particle(const float4 *__restrict__ sh, float h, floa
 }
 
 // particle_update.slang:32-160 for one particle. `aff` holds the velocity
-// gradient on entry and the APIC affine matrix on exit. MODELS is the set of
+// gradient on entry and the APIC affine matrix on exit. No multiply-add
+// contraction here or in the stress helpers: F - I and mu (F F^T - I) are small
+// differences of O(1) values, so each rounding step shows in them; unfused
+// products and sums round like the shader source's separate operations. MODELS is the set of
 // material tags the kernel instance handles (bit t = tag t), so scenes without
 // SVD-based models do not carry that code's register footprint.
 template <uint32_t MODELS>
 __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 &v, float &mass, uint32_t &enabled,
                                        float *aff, float *F, float4 prop, float4 model, const PStatic &st,
                                        uint32_t oid, float *dps) {
+#pragma clang fp contract(off)
     const float dt = P.dt, h = P.h;
     float vl = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
     if (vl > h / dt) {

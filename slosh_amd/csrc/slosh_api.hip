@@ -82,6 +82,27 @@ hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b,
                                   float *fdt, const PassEvents *ev, hipStream_t stream);
 hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, const PassEvents *ev, hipStream_t stream);
 
+struct CompactArgs {
+    float4 *pos[2], *vm[2];
+    float *C[2], *F[2], *vgd[2], *dps[2], *fdt[2];
+    uint32_t *ids[2];
+    float4 *prop[2], *model[2], *dp0[2], *dp1[2];
+    uint32_t *keep_slot, *keep_id, *keep_id2, *tile_sums, *total;
+};
+hipError_t launch_compact(const CompactArgs &a, uint32_t n, uint32_t pcap, uint32_t mode, const uint8_t *dead,
+                          uint32_t lo, uint32_t hi, hipStream_t st);
+struct GatherArgs {
+    float4 *pos, *vm;
+    float *C, *F, *vgd, *dps, *fdt;
+    uint32_t *ids;
+    float4 *s_pos, *s_vm;
+    float *s_C, *s_F, *s_vgd, *s_dps, *s_fdt;
+    uint32_t *rank_id, *tile_sums, *total;
+};
+hipError_t launch_scene_gather(const GatherArgs &a, uint32_t n, uint32_t pcap, uint32_t scene, uint32_t count,
+                               hipStream_t st);
+uint32_t scan_tiles(uint32_t n);
+
 // ---- error plumbing ---------------------------------------------------------
 static thread_local std::string g_last_error;
 
@@ -151,6 +172,12 @@ struct slosh_data {
     PBufs pb[2];
     float *fdt = nullptr;
     float4 *s_prop = nullptr, *s_model = nullptr, *s_dp0 = nullptr, *s_dp1 = nullptr;
+    // destinations of the device-side compaction (swapped with the above)
+    float *fdt2 = nullptr;
+    float4 *s2_prop = nullptr, *s2_model = nullptr, *s2_dp0 = nullptr, *s2_dp1 = nullptr;
+    uint32_t *scan_tmp = nullptr;  // scan tile sums + total
+    uint8_t *dead = nullptr;       // per-scene removal flags [SLOSH_MAX_SCENES]
+    std::vector<uint8_t> dead_host;  // source of the last flag upload (outlives the async copy)
     std::vector<slosh_particle_model> models_host;      // by original index
     std::vector<slosh_particle_properties> props_host;  // by original index
     std::vector<uint32_t> scene_host;                   // by original index
@@ -229,12 +256,18 @@ static int alloc_particle_buffers(slosh_data *d, size_t pcap, bool sand) {
         if (sand) TRY(dalloc(&p.dps, 3 * pcap));
     }
     TRY(dalloc(&d->fdt, 3 * pcap));
+    TRY(dalloc(&d->fdt2, 3 * pcap));
     TRY(dalloc(&d->s_prop, pcap));
     TRY(dalloc(&d->s_model, pcap));
+    TRY(dalloc(&d->s2_prop, pcap));
+    TRY(dalloc(&d->s2_model, pcap));
     if (sand) {
         TRY(dalloc(&d->s_dp0, pcap));
         TRY(dalloc(&d->s_dp1, pcap));
+        TRY(dalloc(&d->s2_dp0, pcap));
+        TRY(dalloc(&d->s2_dp1, pcap));
     }
+    TRY(dalloc(&d->scan_tmp, scan_tiles((uint32_t)pcap) + 4));
     TRY(dalloc(&d->pslot, pcap));
     TRY(dalloc(&d->prank, pcap));
     TRY(dalloc(&d->perm, pcap));
@@ -258,6 +291,12 @@ static void free_particle_buffers(slosh_data *d) {
     dfree(d->s_model);
     dfree(d->s_dp0);
     dfree(d->s_dp1);
+    dfree(d->fdt2);
+    dfree(d->s2_prop);
+    dfree(d->s2_model);
+    dfree(d->s2_dp0);
+    dfree(d->s2_dp1);
+    dfree(d->scan_tmp);
     dfree(d->pslot);
     dfree(d->prank);
     dfree(d->perm);
@@ -693,6 +732,7 @@ static void free_data(slosh_data *d) {
     dfree(d->local_com);
     dfree(d->mats);
     dfree(d->impulses);
+    dfree(d->dead);
     if (d->capture_stream) (void)hipStreamDestroy(d->capture_stream);
     delete d;
 }
@@ -766,7 +806,9 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     bool any_force = false, any_sand = false;
     Staging s;
     TRY(stage_particles(particles, 0, s, &any_force, &any_sand));
-    d->has_sand = any_sand ? 1u : 0u;
+    // Batched data keeps Drucker-Prager state arrays from the start, so scenes of
+    // any material can be appended without reallocating.
+    d->has_sand = (any_sand || d->batched) ? 1u : 0u;
     d->has_force = any_force ? 1u : 0u;
     TRY(alloc_particle_buffers(d, d->pcap, d->has_sand != 0));
     d->models_host.assign(particles->models, particles->models + d->n);
@@ -792,6 +834,7 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     TRY(dalloc(&d->blk_cnt, cap));
     TRY(dalloc(&d->counters, 16));
     HIP_TRY(hipMemset(d->counters, 0, 64));
+    TRY(dalloc(&d->dead, SLOSH_MAX_SCENES + 1));
     TRY(dalloc(&d->tiles, cap * TILE_NODES));
     TRY(dalloc(&d->nodes, cap * NODES_PER_BLOCK));
     d->nbodies = (uint32_t)num_bodies;
@@ -1295,51 +1338,66 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
     return SLOSH_OK;
 }
 
-static int remove_ids(slosh_data *d, hipStream_t st, const std::vector<uint8_t> &keep) {
-    Staging all;
-    bool any_force = false;
-    TRY(gather_all(d, st, all, &any_force));
-    const size_t n = all.n;
-    size_t m = 0;
-    for (size_t i = 0; i < n; i++) m += keep[i];
-    Staging r;
-    r.resize(m);
-    std::vector<slosh_particle_model> models;
-    std::vector<slosh_particle_properties> props;
-    std::vector<uint32_t> scenes;
-    size_t j = 0;
-    for (size_t i = 0; i < n; i++) {
-        if (!keep[i]) continue;
-        for (int c = 0; c < 4; c++) {
-            r.pos[4 * j + c] = all.pos[4 * i + c];
-            r.vm[4 * j + c] = all.vm[4 * i + c];
-            r.prop[4 * j + c] = all.prop[4 * i + c];
-            r.model[4 * j + c] = all.model[4 * i + c];
-            r.dp0[4 * j + c] = all.dp0[4 * i + c];
-            r.dp1[4 * j + c] = all.dp1[4 * i + c];
-        }
-        for (int k = 0; k < 9; k++) {
-            r.C[k * m + j] = all.C[k * n + i];
-            r.F[k * m + j] = all.F[k * n + i];
-        }
-        for (int k = 0; k < 3; k++) {
-            r.dps[k * m + j] = all.dps[k * n + i];
-            r.fdt[k * m + j] = all.fdt[k * n + i];
-        }
-        r.vgd[j] = all.vgd[i];
-        r.ids[j] = (uint32_t)j;
-        models.push_back(d->models_host[i]);
-        props.push_back(d->props_host[i]);
-        scenes.push_back(d->scene_host[i]);
-        j++;
+// Device-side removal (mode 0: the scenes flagged in dead_host; mode 1: ids in
+// [lo, hi)). Kept particles are compacted into the other particle buffer in slot
+// order and their original ids renumbered in order; the host mirrors follow.
+static int device_remove(slosh_data *d, hipStream_t st, uint32_t mode, const std::vector<uint8_t> &dead_host,
+                         size_t lo, size_t hi, const std::vector<uint8_t> &keep_by_id) {
+    const size_t n = d->n;
+    invalidate_graphs(d);
+    if (mode == 0) {
+        HIP_TRY(hipStreamSynchronize(st));  // a previous flag upload may still read dead_host
+        d->dead_host = dead_host;
+        HIP_TRY(hipMemcpyAsync(d->dead, d->dead_host.data(), SLOSH_MAX_SCENES, hipMemcpyHostToDevice, st));
     }
-    d->models_host = models;
-    d->props_host = props;
-    d->scene_host = scenes;
+    const int c = d->cur_p, o = c ^ 1;
+    CompactArgs a{};
+    for (int k = 0; k < 2; k++) {
+        const PBufs &p = d->pb[k == 0 ? c : o];
+        a.pos[k] = p.pos;
+        a.vm[k] = p.vm;
+        a.C[k] = p.C;
+        a.F[k] = p.F;
+        a.vgd[k] = p.vgd;
+        a.dps[k] = p.dps;
+        a.ids[k] = p.ids;
+    }
+    a.fdt[0] = d->fdt;
+    a.fdt[1] = d->fdt2;
+    a.prop[0] = d->s_prop;
+    a.prop[1] = d->s2_prop;
+    a.model[0] = d->s_model;
+    a.model[1] = d->s2_model;
+    a.dp0[0] = d->s_dp0;
+    a.dp0[1] = d->s2_dp0;
+    a.dp1[0] = d->s_dp1;
+    a.dp1[1] = d->s2_dp1;
+    a.keep_slot = d->pslot;
+    a.keep_id = d->prank;
+    a.keep_id2 = d->perm;
+    a.tile_sums = d->scan_tmp;
+    a.total = d->scan_tmp + scan_tiles((uint32_t)d->pcap);
+    HIP_TRY(launch_compact(a, (uint32_t)n, (uint32_t)d->pcap, mode, d->dead, (uint32_t)lo, (uint32_t)hi, st));
+    d->cur_p = o;
+    std::swap(d->fdt, d->fdt2);
+    std::swap(d->s_prop, d->s2_prop);
+    std::swap(d->s_model, d->s2_model);
+    std::swap(d->s_dp0, d->s2_dp0);
+    std::swap(d->s_dp1, d->s2_dp1);
+    size_t m = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (!keep_by_id[i]) continue;
+        d->models_host[m] = d->models_host[i];
+        d->props_host[m] = d->props_host[i];
+        d->scene_host[m] = d->scene_host[i];
+        m++;
+    }
+    d->models_host.resize(m);
+    d->props_host.resize(m);
+    d->scene_host.resize(m);
+    d->n = m;
     d->tag_mask = tag_mask_of(d->models_host);
     update_uniform(d);
-    TRY(rebuild(d, st, d->pcap, r, false));
-    d->has_force = any_force ? 1u : 0u;
     return SLOSH_OK;
 }
 
@@ -1349,21 +1407,107 @@ int slosh_remove_particles(slosh_data *d, slosh_stream stream, size_t start, siz
     if (start == end) return SLOSH_OK;
     std::vector<uint8_t> keep(d->n, 1);
     for (size_t i = start; i < end; i++) keep[i] = 0;
-    return remove_ids(d, (hipStream_t)stream, keep);
+    return device_remove(d, (hipStream_t)stream, 1, {}, start, end, keep);
 }
 
-int slosh_remove_scene(slosh_data *d, slosh_stream stream, uint32_t scene_id, size_t *removed) {
-    if (!d) return fail(SLOSH_E_INVALID, "null data");
+int slosh_remove_scenes(slosh_data *d, slosh_stream stream, const uint32_t *scene_ids, size_t count,
+                        size_t *removed) {
+    if (!d || (!scene_ids && count)) return fail(SLOSH_E_INVALID, "null argument");
+    std::vector<uint8_t> dead(SLOSH_MAX_SCENES, 0);
+    for (size_t k = 0; k < count; k++) {
+        if (scene_ids[k] >= SLOSH_MAX_SCENES) return fail(SLOSH_E_INVALID, "scene id out of range");
+        dead[scene_ids[k]] = 1;
+    }
     std::vector<uint8_t> keep(d->n, 1);
     size_t cnt = 0;
     for (size_t i = 0; i < d->n; i++)
-        if (d->scene_host[i] == scene_id) {
+        if (dead[d->scene_host[i]]) {
             keep[i] = 0;
             cnt++;
         }
     if (removed) *removed = cnt;
     if (cnt == 0) return SLOSH_OK;
-    return remove_ids(d, (hipStream_t)stream, keep);
+    return device_remove(d, (hipStream_t)stream, 0, dead, 0, 0, keep);
+}
+
+int slosh_remove_scene(slosh_data *d, slosh_stream stream, uint32_t scene_id, size_t *removed) {
+    return slosh_remove_scenes(d, stream, &scene_id, 1, removed);
+}
+
+size_t slosh_scene_size(const slosh_data *d, uint32_t scene_id) {
+    if (!d) return 0;
+    size_t c = 0;
+    for (uint32_t s : d->scene_host) c += s == scene_id;
+    return c;
+}
+
+int slosh_read_scene(slosh_data *d, slosh_stream stream, uint32_t scene_id, float *pos4, slosh_kinematics *kin,
+                     slosh_matrix *def_grad, size_t max_count, size_t *count) {
+    if (!d) return fail(SLOSH_E_INVALID, "null data");
+    const size_t m = slosh_scene_size(d, scene_id);
+    if (count) *count = m;
+    if (m == 0) return SLOSH_OK;
+    if (m > max_count) return fail(SLOSH_E_INVALID, "output smaller than the scene");
+    hipStream_t st = (hipStream_t)stream;
+    // Staging: the particle buffer the next substep writes (free between substeps).
+    const PBufs &cur = d->pb[d->cur_p];
+    const PBufs &stg = d->pb[d->cur_p ^ 1];
+    GatherArgs a{};
+    a.pos = cur.pos;
+    a.vm = cur.vm;
+    a.C = cur.C;
+    a.F = cur.F;
+    a.vgd = cur.vgd;
+    a.dps = cur.dps;
+    a.fdt = d->fdt;
+    a.ids = cur.ids;
+    a.s_pos = stg.pos;
+    a.s_vm = stg.vm;
+    a.s_C = stg.C;
+    a.s_F = stg.F;
+    a.s_vgd = stg.vgd;
+    a.s_dps = stg.dps;
+    a.s_fdt = d->fdt2;
+    a.rank_id = d->prank;
+    a.tile_sums = d->scan_tmp;
+    a.total = d->scan_tmp + scan_tiles((uint32_t)d->pcap);
+    HIP_TRY(launch_scene_gather(a, (uint32_t)d->n, (uint32_t)d->pcap, scene_id, (uint32_t)m, st));
+    std::vector<float> pos(4 * m), vm(4 * m), C(9 * m), F(9 * m), vgd(m), fdt(3 * m);
+    HIP_TRY(hipMemcpyAsync(pos.data(), stg.pos, 16 * m, hipMemcpyDeviceToHost, st));
+    if (kin) {
+        HIP_TRY(hipMemcpyAsync(vm.data(), stg.vm, 16 * m, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(C.data(), stg.C, 36 * m, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(vgd.data(), stg.vgd, 4 * m, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(fdt.data(), d->fdt2, 12 * m, hipMemcpyDeviceToHost, st));
+    }
+    if (def_grad) HIP_TRY(hipMemcpyAsync(F.data(), stg.F, 36 * m, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (size_t j = 0; j < m; j++) {
+        if (pos4) {
+            for (int c = 0; c < 3; c++) pos4[4 * j + c] = pos[4 * j + c];
+            pos4[4 * j + 3] = 0.0f;
+        }
+        if (kin) {
+            slosh_kinematics &k = kin[j];
+            std::memset(&k, 0, sizeof k);
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) k.affine.m[r][c] = C[(size_t)(r * 3 + c) * m + j];
+            for (int c = 0; c < 3; c++) {
+                k.velocity[c] = vm[4 * j + c];
+                k.force_dt[c] = fdt[(size_t)c * m + j];
+            }
+            k.vel_grad_det = vgd[j];
+            k.mass = vm[4 * j + 3];
+            k.enabled = f2u(pos[4 * j + 3]) & 1u;
+        }
+        if (def_grad) {
+            slosh_matrix &mm = def_grad[j];
+            std::memset(&mm, 0, sizeof mm);
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) mm.m[r][c] = F[(size_t)(r * 3 + c) * m + j];
+        }
+    }
+    return SLOSH_OK;
 }
 
 int slosh_write_kinematics(slosh_data *d, slosh_stream stream, const slosh_kinematics *kin) {

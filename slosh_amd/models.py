@@ -173,4 +173,15 @@ def particle_arrays(positions, radius, density, model, velocities=None):
 
 
 def concat_particles(parts):
-    return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    """Concatenates particle arrays, keeping each array's exact (padded) record dtype."""
+    out = {}
+    for k in parts[0]:
+        first = parts[0][k]
+        total = sum(p[k].shape[0] for p in parts)
+        arr = np.zeros((total,) + first.shape[1:], dtype=first.dtype)
+        o = 0
+        for p in parts:
+            arr[o:o + p[k].shape[0]] = p[k]
+            o += p[k].shape[0]
+        out[k] = arr
+    return out

@@ -203,7 +203,9 @@ def _particles_struct(particles, scene_ids=None):
     assert keep["kin"].dtype == L.KINEMATICS and keep["props"].dtype == L.PROPERTIES
     keep["models"] = keep["models"].astype(np.uint32, copy=False)
     if scene_ids is not None:
-        keep["scene"] = np.ascontiguousarray(scene_ids, dtype=np.uint32)
+        # A non-null pointer marks the data as batched, also when it starts empty.
+        sid = np.ascontiguousarray(scene_ids, dtype=np.uint32)
+        keep["scene"] = sid if sid.size else np.zeros(1, np.uint32)
     n = keep["pos4"].shape[0]
     st = _lib.Particles(n, keep["pos4"].ctypes.data, keep["kin"].ctypes.data, keep["def_grad"].ctypes.data,
                         keep["props"].ctypes.data, keep["models"].ctypes.data,
@@ -233,6 +235,11 @@ class MpmData:
     def from_scene(cls, pipeline, scene, **kw):
         return cls(pipeline, scene.params, scene.particles, scene.bodies, scene.materials, scene.cell_width,
                    scene.grid_capacity, **kw)
+
+    @classmethod
+    def from_batch(cls, pipeline, batch, **kw):
+        return cls(pipeline, batch.params, batch.particles, batch.bodies, batch.materials, batch.cell_width,
+                   batch.grid_capacity, scene_ids=batch.scene_ids, **kw)
 
     def __del__(self):
         try:
@@ -336,6 +343,27 @@ class MpmData:
         r = C.c_size_t()
         _lib.check(_lib.load().slosh_remove_scene(self.handle, self._s(stream), int(scene_id), C.byref(r)))
         return r.value
+
+    def remove_scenes(self, scene_ids, stream=None):
+        ids = np.ascontiguousarray(scene_ids, dtype=np.uint32)
+        r = C.c_size_t()
+        _lib.check(_lib.load().slosh_remove_scenes(self.handle, self._s(stream), _p(ids), ids.size, C.byref(r)))
+        return r.value
+
+    def scene_size(self, scene_id):
+        return int(_lib.load().slosh_scene_size(self.handle, int(scene_id)))
+
+    def read_scene(self, scene_id, stream=None):
+        """(positions [m, 4], kinematics [m], def_grad [m, 3, 4]) of one scene in creation
+        order, in the formats of positions() / kinematics() / def_grad()."""
+        m = self.scene_size(scene_id)
+        pos = np.zeros((m, 4), np.float32)
+        kin = np.zeros(m, L.KINEMATICS)
+        f = np.zeros((m, 3, 4), np.float32)
+        cnt = C.c_size_t()
+        _lib.check(_lib.load().slosh_read_scene(self.handle, self._s(stream), int(scene_id), _p(pos), _p(kin), _p(f),
+                                                m, C.byref(cnt)))
+        return pos, kin, f
 
     def write_kinematics(self, kin, stream=None):
         kin = np.ascontiguousarray(kin, dtype=L.KINEMATICS)

@@ -20,6 +20,9 @@ Configurations (sizes as in the benchmark definition):
 * ``batched_scene``  -- one scene of the 256-scene batch: N log-uniform in
   [2e3, 2e5], material uniform over NH / FC / DP, E log-uniform [1e4, 1e6],
   nu U[0.2, 0.45], dx = 1/128, the elastic-cube layout scaled to N.
+* ``served_request`` -- one request of the served-scenes stream: a batched-style
+  scene with N log-uniform in [5e3, 2e6] and substeps uniform in [100, 2000];
+  ``poisson_arrivals`` gives the arrival times.
 """
 import numpy as np
 
@@ -169,6 +172,96 @@ def batched_scene(scene_index, seed_base=SEED_BASE + 3, n_override=None):
     parts = M.particle_arrays(pos, np.float32(dx / 4), 1000.0, model)
     bodies, mats = box_walls(M.STICK)
     return Scene(f"batched_{scene_index}", parts, 1.0e-4, dx, 4096, bodies, mats, substeps=200)
+
+
+def served_request(k, seed_base=SEED_BASE + 4, n_range=(5.0e3, 2.0e6), substep_range=(100, 2000)):
+    """Request k of the served-scenes stream: a batched-configuration scene with N
+    log-uniform in n_range and a substep count uniform in substep_range.
+    Returns (scene, substeps)."""
+    u = uniform(seed_base * 7919 + k, 2)
+    n = int(np.exp(np.log(n_range[0]) + u[0] * (np.log(n_range[1]) - np.log(n_range[0]))))
+    steps = int(substep_range[0] + u[1] * (substep_range[1] - substep_range[0] + 1))
+    steps = min(steps, substep_range[1])
+    sc = batched_scene(k, seed_base=seed_base, n_override=n)
+    sc.name = f"request_{k}"
+    return sc, steps
+
+
+def poisson_arrivals(rate, count, seed=SEED_BASE + 4):
+    """Arrival times (s) of a Poisson process of the given rate (exponential gaps)."""
+    u = uniform(seed * 104729 + int(rate * 1000), count)
+    return np.cumsum(-np.log1p(-u) / rate)
+
+
+def stir(scene, seed, rate=2.0):
+    """Gives a scene a smooth nonuniform initial velocity field, v = w x r + A r
+    about its centroid with w and A uniform in [-rate, rate] (1/s), so that the
+    APIC matrices and deformation gradients it develops are well above round-off
+    (a scene falling from rest has a uniform velocity field and C ~ 0).
+    Returns the scene, modified in place."""
+    pos = scene.particles["pos4"][:, :3].astype(np.float64)
+    r = pos - pos.mean(axis=0)
+    u = uniform(seed, 12)
+    w = (2.0 * u[:3] - 1.0) * rate
+    a = ((2.0 * u[3:] - 1.0) * rate).reshape(3, 3)
+    v = np.cross(w, r) + r @ a.T
+    scene.particles["kin"]["velocity"] = v.astype(np.float32)
+    return scene
+
+
+def estimate_blocks(pos, cell_width):
+    """Upper estimate of the active 4x4x4 blocks a particle cloud touches: its
+    bounding box in blocks plus the +1 neighbour shell on every axis."""
+    if len(pos) == 0:
+        return 0
+    lo = np.floor(np.min(pos[:, :3], axis=0) / (4.0 * cell_width)) - 1
+    hi = np.floor(np.max(pos[:, :3], axis=0) / (4.0 * cell_width)) + 2
+    return int(np.prod(hi - lo + 1))
+
+
+def pow2_at_least(v):
+    c = 1
+    while c < v:
+        c <<= 1
+    return c
+
+
+def _values(rec):
+    """Field values of a structured record or array (padding bytes excluded)."""
+    return repr(np.asarray(rec).reshape(-1).tolist())
+
+
+class Batch:
+    """Independent scenes stepped together: every particle of scenes[k] carries
+    scene id ids[k] (its own block-key space); time step, gravity, cell width and
+    colliders are shared, as one launch applies them to every scene."""
+
+    def __init__(self, scene_list, ids=None, grid_capacity=None):
+        assert scene_list, "empty batch"
+        s0 = scene_list[0]
+        for sc in scene_list[1:]:
+            assert _values(sc.params) == _values(s0.params), "scenes of a batch share time step and gravity"
+            assert sc.cell_width == s0.cell_width, "scenes of a batch share the cell width"
+            assert _values(sc.bodies) == _values(s0.bodies) and _values(sc.materials) == _values(s0.materials), \
+                "scenes of a batch share their colliders"
+        self.scenes = list(scene_list)
+        self.ids = np.arange(len(scene_list), dtype=np.uint32) if ids is None else np.asarray(ids, np.uint32)
+        self.counts = np.array([sc.n for sc in scene_list], dtype=np.int64)
+        self.particles = M.concat_particles([sc.particles for sc in scene_list])
+        self.scene_ids = np.repeat(self.ids, self.counts).astype(np.uint32)
+        self.params = s0.params
+        self.cell_width = s0.cell_width
+        self.bodies = s0.bodies
+        self.materials = s0.materials
+        if grid_capacity is None:
+            est = sum(estimate_blocks(sc.particles["pos4"], sc.cell_width) for sc in scene_list)
+            grid_capacity = pow2_at_least(est + est // 4 + 1024)
+        self.grid_capacity = int(grid_capacity)
+        self.name = f"batch of {len(scene_list)}"
+
+    @property
+    def n(self):
+        return int(self.counts.sum())
 
 
 def mixed_test_scene(n_side=20, seed=12345, moving_body=True):

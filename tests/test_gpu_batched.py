@@ -1,0 +1,135 @@
+"""Batched scenes (many independent scenes in one launch) on the GPU.
+
+Each scene of a batch must give what the oracle gives for that scene alone: the
+active-block set and per-block counts bit-exact (blocks compared per scene, keyed
+by the scene-local virtual id) and x, v, affine, F within relative L2 <= 1e-5
+after one substep and <= 1e-3 after 100. Device-side removal must keep the other
+scenes' state bit for bit.
+"""
+import numpy as np
+import pytest
+
+from slosh_amd import pipeline as P
+from slosh_amd import scenes
+from oracle import oracle as O
+from tests.parity import assert_blocks_equal, state_errors
+
+pytestmark = pytest.mark.gpu
+
+TOL = {1: 1.0e-5, 100: 1.0e-3}
+
+
+def small_batch():
+    # Scenes 0..3 of the batched configuration at reduced sizes: neo-Hookean,
+    # fixed-corotated and Drucker-Prager all occur among them.
+    # Stirred (nonuniform initial velocity) so that APIC matrices are not round-off.
+    counts = (3000, 1200, 5000, 2000)
+    scs = [scenes.stir(scenes.batched_scene(i, n_override=n), 77 + i) for i, n in enumerate(counts)]
+    tags = {int(sc.particles["models"][0][0]) for sc in scs}
+    assert len(tags) >= 2
+    return scenes.Batch(scs, ids=[0, 7, 3, 12])
+
+
+@pytest.mark.parametrize("steps", [1, 100])
+def test_batch_matches_each_scene_alone(gpu_pipeline, steps):
+    pl, st = gpu_pipeline
+    b = small_batch()
+    data = P.MpmData.from_batch(pl, b)
+    for _ in range(steps):
+        pl.launch_step(data, st)
+    data.check_errors(st)
+    blocks, bscene = data.blocks(st), data.block_scenes(st)
+    for sc, sid in zip(b.scenes, b.ids):
+        orc = O.Oracle(sc, svd_mode=1)
+        orc.step(steps)
+        assert_blocks_equal(blocks[bscene == sid], orc.blocks())
+        pos, kin, f = data.read_scene(sid, st)
+        errs = state_errors((pos, kin, f), (orc.positions(), orc.kinematics(), orc.def_grad()),
+                            sc.particles["pos4"])
+        print(f"\n[batched scene {sid} (n={sc.n}) after {steps}] " +
+              " ".join(f"{k}={v:.3e}" for k, v in errs.items() if k != "enabled_diff"))
+        assert errs["enabled_diff"] == 0
+        for k in ("x", "v", "affine", "F"):
+            assert errs[k] <= TOL[steps], f"scene {sid} {k}: {errs[k]:.3e}"
+
+
+def test_read_scene_matches_full_readback(gpu_pipeline):
+    pl, st = gpu_pipeline
+    b = small_batch()
+    data = P.MpmData.from_batch(pl, b)
+    for _ in range(3):
+        pl.launch_step(data, st)
+    pos, kin, f = data.positions(st), data.kinematics(st), data.def_grad(st)
+    sid_of = data.scene_ids(st)
+    assert np.array_equal(sid_of, b.scene_ids)
+    for sid in b.ids:
+        m = sid_of == sid
+        p2, k2, f2 = data.read_scene(sid, st)
+        assert np.array_equal(p2[:, :3], pos[m, :3])
+        assert np.array_equal(k2, kin[m])
+        assert np.array_equal(f2, f[m])
+    assert data.scene_size(999) == 0
+
+
+def test_device_remove_scenes_keeps_others_exact(gpu_pipeline):
+    pl, st = gpu_pipeline
+    b = small_batch()
+    data = P.MpmData.from_batch(pl, b)
+    for _ in range(4):
+        pl.launch_step(data, st)
+    pos, kin, f = data.positions(st), data.kinematics(st), data.def_grad(st)
+    models, props = data.models(st), data.properties(st)
+    keep = (b.scene_ids != 7) & (b.scene_ids != 12)
+    removed = data.remove_scenes([7, 12], st)
+    assert removed == int((~keep).sum())
+    assert data.n == int(keep.sum())
+    assert np.array_equal(data.scene_ids(st), b.scene_ids[keep])
+    assert np.array_equal(data.positions(st)[:, :3], pos[keep, :3])
+    assert np.array_equal(data.kinematics(st), kin[keep])
+    assert np.array_equal(data.def_grad(st), f[keep])
+    assert np.array_equal(data.models(st), models[keep])
+    assert np.array_equal(data.properties(st), props[keep])
+    # The remaining scenes continue as if alone: compare with a batch of only them.
+    ref = P.MpmData.from_batch(pl, scenes.Batch([b.scenes[0], b.scenes[2]], ids=[0, 3]))
+    for _ in range(4):
+        pl.launch_step(ref, st)
+    for _ in range(3):
+        pl.launch_step(data, st)
+        pl.launch_step(ref, st)
+    data.check_errors(st)
+    for sid in (0, 3):
+        a, r = data.read_scene(sid, st), ref.read_scene(sid, st)
+        errs = state_errors(a, r, r[0])
+        assert max(errs[k] for k in ("x", "v", "affine", "F")) <= 1e-5, errs
+
+
+def test_shift_remove_on_device_then_step(gpu_pipeline):
+    pl, st = gpu_pipeline
+    sc = scenes.elastic_cube(n_side=12)
+    data = P.MpmData.from_scene(pl, sc)
+    for _ in range(2):
+        pl.launch_step(data, st)
+    pos = data.positions(st)
+    data.shift_remove(100, 900)
+    assert data.n == sc.n - 800
+    assert np.array_equal(data.positions(st)[:, :3], np.concatenate([pos[:100, :3], pos[900:, :3]]))
+    pl.launch_steps_graph(data, 4, st)
+    data.check_errors(st)
+
+
+def test_append_scene_to_running_batch(gpu_pipeline):
+    pl, st = gpu_pipeline
+    b = small_batch()
+    data = P.MpmData.from_batch(pl, b, particle_capacity=b.n + 4000)
+    for _ in range(2):
+        pl.launch_step(data, st)
+    extra = scenes.stir(scenes.batched_scene(9, n_override=3500), 91)
+    data.append(extra.particles, scene_ids=np.full(extra.n, 40, np.uint32), stream=st)
+    for _ in range(5):
+        pl.launch_step(data, st)
+    data.check_errors(st)
+    orc = O.Oracle(extra, svd_mode=1)
+    orc.step(5)
+    errs = state_errors(data.read_scene(40, st), (orc.positions(), orc.kinematics(), orc.def_grad()),
+                        extra.particles["pos4"])
+    assert max(errs[k] for k in ("x", "v", "affine", "F")) <= 1e-4, errs

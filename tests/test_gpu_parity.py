@@ -78,6 +78,14 @@ def test_parity_elastic_cube(gpu_pipeline, steps):
 
 
 @pytest.mark.parametrize("steps", [1, 100])
+def test_parity_elastic_cube_stirred(gpu_pipeline, steps):
+    # The same cube with a nonuniform initial velocity field: APIC gated too.
+    sc = scenes.stir(scenes.elastic_cube(), 4242)
+    data, orc = run_both(sc, steps, gpu_pipeline)
+    check_state(sc, data, orc, gpu_pipeline[1], steps, diag=("dx",))
+
+
+@pytest.mark.parametrize("steps", [1, 100])
 def test_parity_mixed(gpu_pipeline, steps):
     sc = scenes.mixed_test_scene()
     data, orc = run_both(sc, steps, gpu_pipeline)
