@@ -99,6 +99,7 @@ def run_plain(args):
     data = P.MpmData.from_scene(pl, scene)
     # warm-up (also sorts particle storage by block)
     pl.launch_step(data, st)
+    tuning = data.autotune(pl, st, trial_steps=10) if args.autotune else data.tuning
     pl.launch_steps_graph(data, max(args.warmup - 1, 0), st)
     st.synchronize()
     data.check_errors(st)
@@ -150,6 +151,7 @@ def run_plain(args):
             "dt": float(scene.params["dt"]),
             "active_blocks": int(n_blk),
             "launch": f"hipGraph of {args.steps} substeps",
+            "tuning": tuning,
         },
         "roofline": {
             "bound": "hbm",
@@ -182,6 +184,7 @@ def run_batched(args):
     st = P.Stream()
     data = P.MpmData.from_batch(pl, batch)
     pl.launch_step(data, st)
+    tuning = data.autotune(pl, st, trial_steps=10) if args.autotune else data.tuning
     pl.launch_steps_graph(data, max(args.warmup - 1, 0), st)
     st.synchronize()
     data.check_errors(st)
@@ -223,6 +226,7 @@ def run_batched(args):
         "steps": args.batch_steps,
         "ms_per_step": ms / args.batch_steps,
         "active_blocks": int(n_blk),
+        "tuning": tuning,
         "per_scene_loop": loop,
         "per_scene_loop_ms_per_step": loop_ms / args.batch_steps,
         "speedup_vs_per_scene_loop": batched / loop,
@@ -246,7 +250,8 @@ def run_served(args):
         done = srv.serve(reqs, realtime=True)
         summ = SV.latency_summary(done)
         summ.update({"rate_per_s": rate, "peak_particles_in_flight": srv.peak_in_flight,
-                     "device_bytes": srv.device_bytes, "chunks": srv.chunks})
+                     "device_bytes": srv.device_bytes, "chunks": srv.chunks,
+                     "host_seconds": {k: round(v, 4) for k, v in srv.seconds.items()}})
         srv.data.close()
         out.append(summ)
     # The plain loop at the highest rate (the most overlap to gain).
@@ -274,6 +279,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32, help="requests per served run (0 = skip)")
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[10.0, 50.0, 200.0])
     ap.add_argument("--hbm-budget", type=float, default=200e9)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shape instead of autotuning it before timing")
     args = ap.parse_args()
     args.steps += args.steps & 1  # the captured graph holds an even number of substeps
     if args.gpus != 1:

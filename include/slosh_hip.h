@@ -281,6 +281,25 @@ int slosh_write_kinematics(slosh_data *data, slosh_stream stream, const slosh_ki
 
 int slosh_device_view_get(slosh_data *data, slosh_device_view *out);
 
+/* ---- Launch shape (no reference counterpart: the wgpu pipeline fixes its
+ * workgroup sizes in the shaders). p2g_chunk: particles staged in LDS per P2G
+ * round (256, 512, 1024); g2p_threads: threads per G2P/particle-update
+ * workgroup (128, 256); block_wgs: workgroups of the per-block kernels, which
+ * loop over the active blocks (64..65536). Results do not depend on it beyond
+ * summation order within a cell. slosh_autotune times `trial_steps` substeps of
+ * a private copy of `data` per candidate (coordinate search, best of two
+ * replays), keeps the fastest in `data` and returns it in `out` (may be null);
+ * `data` itself does not advance. */
+typedef struct slosh_tuning {
+    uint32_t p2g_chunk;
+    uint32_t g2p_threads;
+    uint32_t block_wgs;
+} slosh_tuning;
+int slosh_set_tuning(slosh_data *data, const slosh_tuning *tuning);
+int slosh_get_tuning(const slosh_data *data, slosh_tuning *out);
+int slosh_autotune(slosh_pipeline *pipeline, slosh_data *data, slosh_stream stream, uint32_t trial_steps,
+                   slosh_tuning *out);
+
 /* ---- Timing: per-pass GPU timestamps (GpuTimestamps; pass labels of
  * src/pipeline.rs:588-757: grid_sort, p2g, grid_update, g2p, particles_update,
  * integrate_bodies). The timestamps are written by the dispatch packets of each

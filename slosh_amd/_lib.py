@@ -22,7 +22,7 @@ EXPORTS = [
     "slosh_timestamps_reset", "slosh_stream_create", "slosh_stream_destroy", "slosh_stream_synchronize",
     "slosh_event_create", "slosh_event_destroy", "slosh_event_record", "slosh_event_elapsed_ms",
     "slosh_device_count", "slosh_last_error", "slosh_version", "slosh_graph_capture", "slosh_graph_launch",
-    "slosh_graph_destroy",
+    "slosh_graph_destroy", "slosh_set_tuning", "slosh_get_tuning", "slosh_autotune",
 ]
 
 
@@ -30,6 +30,13 @@ class SloshError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
         self.code = code
+
+
+class Tuning(C.Structure):
+    _fields_ = [("p2g_chunk", C.c_uint32), ("g2p_threads", C.c_uint32), ("block_wgs", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 class Kernels(C.Structure):
@@ -100,6 +107,9 @@ def load():
         "slosh_remove_particles": ([vp, vp, sz, sz], i32),
         "slosh_remove_scene": ([vp, vp, u32, vp], i32),
         "slosh_remove_scenes": ([vp, vp, vp, sz, vp], i32),
+        "slosh_set_tuning": ([vp, vp], i32),
+        "slosh_get_tuning": ([vp, vp], i32),
+        "slosh_autotune": ([vp, vp, vp, u32, vp], i32),
         "slosh_scene_size": ([vp, u32], sz),
         "slosh_read_scene": ([vp, vp, u32, vp, vp, vp, sz, vp], i32),
         "slosh_write_kinematics": ([vp, vp, vp], i32),

@@ -76,26 +76,58 @@ __global__ __launch_bounds__(256) void k_touch_neighbors(uint32_t *__restrict__ 
     }
 }
 
+// Dense header ids for the occupied slots. Each workgroup takes a run of
+// COMPACT_SLOTS slots, ranks its occupied ones with a workgroup scan and reserves
+// their ids with one atomic, so the shared counter sees cap / COMPACT_SLOTS
+// atomics instead of one per wavefront.
+constexpr uint32_t COMPACT_ITEMS = 8;
+constexpr uint32_t COMPACT_SLOTS = 256 * COMPACT_ITEMS;
+
 __global__ __launch_bounds__(256) void k_compact(const uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                  uint32_t cap, uint32_t *__restrict__ hdr_key,
                                                  uint32_t *__restrict__ cell_cnt, uint32_t *__restrict__ counters) {
-    for (uint32_t base = blockIdx.x * blockDim.x; base < cap; base += gridDim.x * blockDim.x) {
-        uint32_t s = base + threadIdx.x;
-        uint32_t k = s < cap ? keys[s] : SLOSH_NONE;
-        bool occ = k != SLOSH_NONE;
-        uint64_t m = __ballot(occ);
-        uint32_t cnt = (uint32_t)__popcll(m);
-        uint32_t wbase = 0u;
-        if (lane_id() == 0 && cnt) wbase = atomicAdd(&counters[CTR_NUM_BLOCKS], cnt);
-        wbase = __shfl(wbase, 0);
-        if (occ) {
-            uint32_t hid = wbase + (uint32_t)__popcll(m & lanemask_lt());
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wg_base;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+    for (uint32_t base = blockIdx.x * COMPACT_SLOTS; base < cap; base += gridDim.x * COMPACT_SLOTS) {
+        uint32_t k[COMPACT_ITEMS];
+        uint32_t mine = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < COMPACT_ITEMS; i++) {
+            const uint32_t s = base + i * 256u + tid;
+            k[i] = s < cap ? keys[s] : SLOSH_NONE;
+            mine += k[i] != SLOSH_NONE ? 1u : 0u;
+        }
+        // Workgroup exclusive scan of the per-thread counts.
+        uint32_t x = mine;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d);
+            if ((int)lane >= d) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < 4; w++) {
+            before += w < wave ? wsum[w] : 0u;
+            total += wsum[w];
+        }
+        if (tid == 0) wg_base = total ? atomicAdd(&counters[CTR_NUM_BLOCKS], total) : 0u;
+        __syncthreads();
+        uint32_t hid = wg_base + before + x - mine;
+#pragma unroll
+        for (uint32_t i = 0; i < COMPACT_ITEMS; i++) {
+            if (k[i] == SLOSH_NONE) continue;
+            const uint32_t s = base + i * 256u + tid;
             vals[s] = hid;
-            hdr_key[hid] = k;
+            hdr_key[hid] = k[i];
             uint4 *cc = reinterpret_cast<uint4 *>(cell_cnt + (size_t)hid * NODES_PER_BLOCK);
 #pragma unroll
             for (int j = 0; j < 16; j++) cc[j] = make_uint4(0u, 0u, 0u, 0u);
+            hid++;
         }
+        __syncthreads();  // wsum / wg_base are reused by the next run
     }
 }
 
@@ -173,47 +205,89 @@ __global__ __launch_bounds__(256) void k_block_prepare(const uint32_t *__restric
 
 // Exclusive scan of the block totals in one workgroup of 1024 threads (4
 // consecutive blocks per thread per pass): blk_start[b] = sum of blk_cnt[< b].
+// Exclusive scan of blk_cnt[base, base + 4096) by one workgroup of 1024 threads
+// (4 consecutive blocks per thread), written to blk_start with `carry` added;
+// returns the tile's total. Every thread of the workgroup must call it.
+__device__ inline uint32_t scan_block_tile(const uint32_t *__restrict__ blk_cnt, uint32_t *__restrict__ blk_start,
+                                           uint32_t base, uint32_t B, uint32_t carry, uint32_t *wsum) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t idx = base + tid * 4u;
+    const uint32_t v0 = idx < B ? blk_cnt[idx] : 0u;
+    const uint32_t v1 = idx + 1 < B ? blk_cnt[idx + 1] : 0u;
+    const uint32_t v2 = idx + 2 < B ? blk_cnt[idx + 2] : 0u;
+    const uint32_t v3 = idx + 3 < B ? blk_cnt[idx + 3] : 0u;
+    const uint32_t t = v0 + v1 + v2 + v3;
+    uint32_t x = t;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d);
+        if ((int)lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t w = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            uint32_t y = __shfl_up(w, d);
+            if ((int)lane >= d) w += y;
+        }
+        if (lane < 16) wsum[lane] = w;
+    }
+    __syncthreads();
+    const uint32_t e = carry + (wave > 0 ? wsum[wave - 1] : 0u) + x - t;
+    if (idx < B) blk_start[idx] = e;
+    if (idx + 1 < B) blk_start[idx + 1] = e + v0;
+    if (idx + 2 < B) blk_start[idx + 2] = e + v0 + v1;
+    if (idx + 3 < B) blk_start[idx + 3] = e + v0 + v1 + v2;
+    const uint32_t total = wsum[15];
+    __syncthreads();
+    return total;
+}
+
+// Exclusive scan of the block totals in one workgroup: blk_start[b] = sum of
+// blk_cnt[< b] (used while the capacity is at most a few tiles).
 __global__ __launch_bounds__(1024) void k_block_scan(const uint32_t *__restrict__ blk_cnt,
                                                      uint32_t *__restrict__ blk_start,
                                                      uint32_t *__restrict__ counters) {
     __shared__ uint32_t wsum[16];
     const uint32_t B = counters[CTR_NUM_BLOCKS];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < B; base += 4096u) {
-        const uint32_t idx = base + tid * 4u;
-        uint32_t v0 = idx < B ? blk_cnt[idx] : 0u;
-        uint32_t v1 = idx + 1 < B ? blk_cnt[idx + 1] : 0u;
-        uint32_t v2 = idx + 2 < B ? blk_cnt[idx + 2] : 0u;
-        uint32_t v3 = idx + 3 < B ? blk_cnt[idx + 3] : 0u;
-        const uint32_t t = v0 + v1 + v2 + v3;
-        uint32_t x = t;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t y = __shfl_up(x, d);
-            if ((int)lane >= d) x += y;
-        }
-        if (lane == 63) wsum[wave] = x;
-        __syncthreads();
-        if (wave == 0) {
-            uint32_t w = lane < 16 ? wsum[lane] : 0u;
-#pragma unroll
-            for (int d = 1; d < 16; d <<= 1) {
-                uint32_t y = __shfl_up(w, d);
-                if ((int)lane >= d) w += y;
-            }
-            if (lane < 16) wsum[lane] = w;
-        }
-        __syncthreads();
-        const uint32_t e = carry + (wave > 0 ? wsum[wave - 1] : 0u) + x - t;
-        if (idx < B) blk_start[idx] = e;
-        if (idx + 1 < B) blk_start[idx + 1] = e + v0;
-        if (idx + 2 < B) blk_start[idx + 2] = e + v0 + v1;
-        if (idx + 3 < B) blk_start[idx + 3] = e + v0 + v1 + v2;
-        carry += wsum[15];
-        __syncthreads();
+    for (uint32_t base = 0; base < B; base += 4096u) carry += scan_block_tile(blk_cnt, blk_start, base, B, carry, wsum);
+    if (threadIdx.x == 0) counters[CTR_SORTED_ALLOC] = carry;
+}
+
+// Three-phase scan for large block counts: tiles of 4096 blocks scanned in
+// parallel, then the tile totals, then the tile offsets added.
+__global__ __launch_bounds__(1024) void k_block_scan_tiles(const uint32_t *__restrict__ blk_cnt,
+                                                           uint32_t *__restrict__ blk_start,
+                                                           uint32_t *__restrict__ tile_sums,
+                                                           const uint32_t *__restrict__ counters) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    for (uint32_t t = blockIdx.x; t * 4096u < B; t += gridDim.x) {
+        const uint32_t total = scan_block_tile(blk_cnt, blk_start, t * 4096u, B, 0u, wsum);
+        if (threadIdx.x == 0) tile_sums[t] = total;
     }
-    if (tid == 0) counters[CTR_SORTED_ALLOC] = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_block_scan_top(uint32_t *__restrict__ tile_sums,
+                                                         uint32_t *__restrict__ counters) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    const uint32_t ntiles = (B + 4095u) / 4096u;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += 4096u)
+        carry += scan_block_tile(tile_sums, tile_sums, base, ntiles, carry, wsum);
+    if (threadIdx.x == 0) counters[CTR_SORTED_ALLOC] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_block_scan_add(uint32_t *__restrict__ blk_start,
+                                                        const uint32_t *__restrict__ tile_sums,
+                                                        const uint32_t *__restrict__ counters) {
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    for (uint32_t b = 4096u + blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x)
+        blk_start[b] += tile_sums[b >> 12];
 }
 
 __global__ __launch_bounds__(256) void k_scatter(const uint32_t *__restrict__ pslot, const uint32_t *__restrict__ prank,
@@ -235,6 +309,7 @@ struct GridBuffers {
     uint32_t *keys, *vals, *prev_keys, *prev_vals;
     uint32_t *hdr_key, *nbr, *cell_cnt, *cell_start, *blk_start, *blk_cnt;
     uint32_t *pslot, *prank, *perm, *counters;
+    uint32_t *tile_sums;  // ceil(cap / 4096) words
 };
 
 static inline uint32_t div_up(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -262,7 +337,8 @@ hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n
     }
     launch_kernel(f, k_touch_neighbors, dim3(min_u(div_up((uint64_t)cap * 8u, 256), 8192)), dim3(256), 0, stream,
                   g.keys, g.vals, cap, batched, g.counters);
-    launch_kernel(nullptr, k_compact, dim3(slot_grid), dim3(256), 0, stream, (const uint32_t *)g.keys, g.vals, cap,
+    launch_kernel(nullptr, k_compact, dim3(min_u(div_up(cap, COMPACT_SLOTS), 2048)), dim3(256), 0, stream,
+                  (const uint32_t *)g.keys, g.vals, cap,
                   g.hdr_key, g.cell_cnt, g.counters);
     if (n > 0)
         launch_kernel(nullptr, k_count, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched,
@@ -273,8 +349,17 @@ hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n
                   g.cell_start, g.blk_cnt, (const uint32_t *)g.hdr_key, g.nbr, (const uint32_t *)g.keys,
                   (const uint32_t *)g.vals, (const uint32_t *)g.prev_keys, (const uint32_t *)g.prev_vals, cap, batched,
                   (const uint32_t *)g.counters);
-    launch_kernel(n > 0 ? nullptr : (ev ? &last : nullptr), k_block_scan, dim3(1), dim3(1024), 0, stream,
-                  (const uint32_t *)g.blk_cnt, g.blk_start, g.counters);
+    const PassEvents *scan_ev = n > 0 ? nullptr : (ev ? &last : nullptr);
+    if (cap <= 16384u) {
+        launch_kernel(scan_ev, k_block_scan, dim3(1), dim3(1024), 0, stream, (const uint32_t *)g.blk_cnt, g.blk_start,
+                      g.counters);
+    } else {
+        launch_kernel(nullptr, k_block_scan_tiles, dim3(min_u(div_up(cap, 4096), 1024)), dim3(1024), 0, stream,
+                      (const uint32_t *)g.blk_cnt, g.blk_start, g.tile_sums, (const uint32_t *)g.counters);
+        launch_kernel(nullptr, k_block_scan_top, dim3(1), dim3(1024), 0, stream, g.tile_sums, g.counters);
+        launch_kernel(scan_ev, k_block_scan_add, dim3(min_u(div_up(cap, 256), 4096)), dim3(256), 0, stream,
+                      g.blk_start, (const uint32_t *)g.tile_sums, (const uint32_t *)g.counters);
+    }
     if (n > 0)
         launch_kernel(ev ? &last : nullptr, k_scatter, dim3(part_grid), dim3(256), 0, stream,
                       (const uint32_t *)g.pslot, (const uint32_t *)g.prank, (const uint32_t *)g.cell_start,

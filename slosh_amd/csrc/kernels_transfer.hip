@@ -87,6 +87,7 @@ __device__ void svd3(const T *a, T *u, T *sig, T *v) {
 #pragma unroll
         for (int j = 0; j < 3; j++) s[i][j] = a[0 * 3 + i] * a[0 * 3 + j] + a[1 * 3 + i] * a[1 * 3 + j] + a[2 * 3 + i] * a[2 * 3 + j];
     T vv[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll 1
     for (int sweep = 0; sweep < 12; sweep++) {
         T off = s[0][1] * s[0][1] + s[0][2] * s[0][2] + s[1][2] * s[1][2];
         T dg = s[0][0] * s[0][0] + s[1][1] * s[1][1] + s[2][2] * s[2][2];
@@ -264,36 +265,125 @@ __device__ inline void dp_project(float *state, const float4 hp, float lambda, f
 // assembly and the Drucker-Prager return mapping run in double and are rounded
 // once. In FP32, F - R and log(sigma) near sigma = 1 lose ~5 digits to
 // cancellation, which is the size of the parity tolerance.
+// Orthogonal polar factor r of a (a = r s, s symmetric positive definite) by
+// Newton's iteration x <- (g x + x^-T / g) / 2 with determinant scaling
+// g = det(x)^(-1/3) while det(x) is far from 1 (N. J. Higham, "Computing the
+// polar decomposition -- with applications", SIAM J. Sci. Stat. Comput. 7(4),
+// 1986). For det(a) > 0 this is U V^T of the SVD a = U diag(sig) V^T. Returns
+// false (r undefined) if det(a) <= 0 or it does not converge.
+__device__ inline bool polar_f64(const double *a, double *r) {
+    double x[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) x[k] = a[k];
+    bool close = false;
+#pragma unroll 1
+    for (int it = 0; it < 24; it++) {
+        double c[9];
+        c[0] = x[4] * x[8] - x[5] * x[7];
+        c[1] = x[5] * x[6] - x[3] * x[8];
+        c[2] = x[3] * x[7] - x[4] * x[6];
+        c[3] = x[2] * x[7] - x[1] * x[8];
+        c[4] = x[0] * x[8] - x[2] * x[6];
+        c[5] = x[1] * x[6] - x[0] * x[7];
+        c[6] = x[1] * x[5] - x[2] * x[4];
+        c[7] = x[2] * x[3] - x[0] * x[5];
+        c[8] = x[0] * x[4] - x[1] * x[3];
+        const double det = x[0] * c[0] + x[1] * c[1] + x[2] * c[2];
+        if (!(det > 0.0)) return false;
+        const double g = fabs(det - 1.0) > 1.0e-3 ? 1.0 / cbrt(det) : 1.0;
+        const double h = 0.5 / (g * det);
+        double diff = 0.0, norm = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const double nx = 0.5 * g * x[k] + h * c[k];
+            diff = fmax(diff, fabs(nx - x[k]));
+            norm = fmax(norm, fabs(nx));
+            x[k] = nx;
+        }
+        // Quadratic convergence: once a step is below 1e-9 the next one is at round-off.
+        if (close) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) r[k] = x[k];
+            return true;
+        }
+        close = diff <= 1.0e-9 * norm;
+    }
+    return false;
+}
+
+__device__ inline double det3d(const double *a) {
+    return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+
+// Singular values of a, descending, with sig[2] carrying the sign of det(a) (the
+// convention of svd3), from the eigenvalues of a^T a by the trigonometric closed
+// form for symmetric 3x3 matrices (O. K. Smith, "Eigenvalues of a symmetric 3 x 3
+// matrix", Commun. ACM 4(4), 1961).
+__device__ inline void singular_values_f64(const double *a, double *sig) {
+    double c00 = a[0] * a[0] + a[3] * a[3] + a[6] * a[6];
+    double c11 = a[1] * a[1] + a[4] * a[4] + a[7] * a[7];
+    double c22 = a[2] * a[2] + a[5] * a[5] + a[8] * a[8];
+    double c01 = a[0] * a[1] + a[3] * a[4] + a[6] * a[7];
+    double c02 = a[0] * a[2] + a[3] * a[5] + a[6] * a[8];
+    double c12 = a[1] * a[2] + a[4] * a[5] + a[7] * a[8];
+    double l0, l1, l2;
+    const double p1 = c01 * c01 + c02 * c02 + c12 * c12;
+    if (p1 == 0.0) {
+        l0 = fmax(c00, fmax(c11, c22));
+        l2 = fmin(c00, fmin(c11, c22));
+        l1 = c00 + c11 + c22 - l0 - l2;
+    } else {
+        const double q = (c00 + c11 + c22) / 3.0;
+        const double d0 = c00 - q, d1 = c11 - q, d2 = c22 - q;
+        const double p = sqrt((d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * p1) / 6.0);
+        const double ip = 1.0 / p;
+        const double b00 = d0 * ip, b11 = d1 * ip, b22 = d2 * ip, b01 = c01 * ip, b02 = c02 * ip, b12 = c12 * ip;
+        double hr = 0.5 * (b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) + b02 * (b01 * b12 - b11 * b02));
+        hr = fmin(1.0, fmax(-1.0, hr));
+        const double phi = acos(hr) / 3.0;
+        l0 = q + 2.0 * p * cos(phi);
+        l2 = q + 2.0 * p * cos(phi + 2.0943951023931954923);  // + 2 pi / 3
+        l1 = 3.0 * q - l0 - l2;
+    }
+    sig[0] = sqrt(fmax(l0, 0.0));
+    sig[1] = sqrt(fmax(l1, 0.0));
+    sig[2] = copysign(sqrt(fmax(l2, 0.0)), det3d(a));
+}
+
+// linear_elasticity.slang:135-148 in FP64: tau = 2 mu (F - R) F^T + lambda (J - 1) J I
+// with R the polar rotation (U V^T of the SVD), J = det F.
 __device__ inline void fc_stress_f64(float lambda, float mu, const float *f32, float *st) {
-    double f[9], u[9], s[3], v[9], rc[9];
+    double f[9], rc[9], j;
 #pragma unroll
     for (int k = 0; k < 9; k++) f[k] = f32[k];
-    svd3<double>(f, u, s, v);
-    double j = s[0] * s[1] * s[2];
-    double sm1[3] = {s[0] - 1.0, s[1] - 1.0, s[2] - 1.0};
-    svd_recompose<double>(u, sm1, v, rc);
+    double r[9];
+    if (polar_f64(f, r)) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) rc[k] = f[k] - r[k];
+        j = det3d(f);
+    } else {
+        double u[9], s[3], v[9];
+        svd3<double>(f, u, s, v);
+        j = s[0] * s[1] * s[2];
+        double sm1[3] = {s[0] - 1.0, s[1] - 1.0, s[2] - 1.0};
+        svd_recompose<double>(u, sm1, v, rc);
+    }
     const double mu2 = 2.0 * (double)mu;
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            double r = mu2 * (f[0 * 3 + i] * rc[0 * 3 + k] + f[1 * 3 + i] * rc[1 * 3 + k] + f[2 * 3 + i] * rc[2 * 3 + k]);
-            if (i == k) r += (double)lambda * (j - 1.0) * j;
-            st[i * 3 + k] = (float)r;
+            double r2 = mu2 * (f[0 * 3 + i] * rc[0 * 3 + k] + f[1 * 3 + i] * rc[1 * 3 + k] + f[2 * 3 + i] * rc[2 * 3 + k]);
+            if (i == k) r2 += (double)lambda * (j - 1.0) * j;
+            st[i * 3 + k] = (float)r2;
         }
 }
 
-__device__ inline void dp_project_f64(float *state, const float4 hp, float lambda_f, float mu_f, float *f32) {
-    if (lambda_f == 0.0f) return;
-    const double lambda = lambda_f, mu = mu_f;
-    double f[9], u[9], s[3], v[9], nf[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) f[k] = f32[k];
+// drucker_prager.slang:144-200 in FP64 with the full SVD (cone projection).
+__device__ inline void dp_project_f64_svd(float *state, const float4 hp, double lambda, double mu, double alpha,
+                                          const double *f, float *f32) {
+    double u[9], s[3], v[9], nf[9];
     svd3<double>(f, u, s, v);
-    double q = state[1];
-    double angle = (double)hp.x + ((double)hp.y * q - (double)hp.w) * exp(-(double)hp.z * q);
-    double sa = sin(angle);
-    double alpha = sqrt(2.0 / 3.0) * (2.0 * sa) / (3.0 - sa);
     const double d = 3.0;
     double strain[3], dev[3], ns[3], hard;
 #pragma unroll
@@ -320,6 +410,48 @@ __device__ inline void dp_project_f64(float *state, const float4 hp, float lambd
     svd_recompose<double>(u, ns, v, nf);
 #pragma unroll
     for (int k = 0; k < 9; k++) f32[k] = (float)nf[k];
+}
+
+// drucker_prager.slang:144-200 in FP64. The singular values decide the case; only
+// the cone projection needs U and V (full SVD). Inside the cone nothing changes;
+// under tension F becomes its polar rotation U V^T.
+__device__ inline void dp_project_f64(float *state, const float4 hp, float lambda_f, float mu_f, float *f32) {
+    if (lambda_f == 0.0f) return;
+    const double lambda = lambda_f, mu = mu_f;
+    double f[9], s[3];
+#pragma unroll
+    for (int k = 0; k < 9; k++) f[k] = f32[k];
+    double q = state[1];
+    double angle = (double)hp.x + ((double)hp.y * q - (double)hp.w) * exp(-(double)hp.z * q);
+    double sa = sin(angle);
+    double alpha = sqrt(2.0 / 3.0) * (2.0 * sa) / (3.0 - sa);
+    singular_values_f64(f, s);
+    const double d = 3.0;
+    double strain[3], dev[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) strain[i] = log(s[i]) + (double)state[2] / d;
+    double tr = strain[0] + strain[1] + strain[2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) dev[i] = strain[i] - tr / d;
+    if (tr > 0.0 || (dev[0] == 0.0 && dev[1] == 0.0 && dev[2] == 0.0)) {
+        double r[9];
+        if (!polar_f64(f, r)) {
+            dp_project_f64_svd(state, hp, lambda, mu, alpha, f, f32);
+            return;
+        }
+        const double hard = sqrt(strain[0] * strain[0] + strain[1] * strain[1] + strain[2] * strain[2]);
+        const double prev_det = s[0] * s[1] * s[2];
+        state[0] = (float)((double)state[0] * prev_det / 1.0);
+        state[2] = (float)((double)state[2] + log(prev_det) - log(1.0));
+        state[1] = (float)((double)state[1] + hard);
+#pragma unroll
+        for (int k = 0; k < 9; k++) f32[k] = (float)r[k];
+        return;
+    }
+    double dn = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+    double gamma = dn + (d * lambda + 2.0 * mu) / (2.0 * mu) * tr * alpha;
+    if (gamma <= 0.0) return;
+    dp_project_f64_svd(state, hp, lambda, mu, alpha, f, f32);
 }
 
 // ---------------------------------------------------------------------------
@@ -661,11 +793,14 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
     const uint32_t tbase = lx + 6 * ly + 36 * (lz + (int)kz);
     const float h = P.h, inv_h = P.inv_h;
     for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
+        const uint32_t b0 = blk_start[hid], nb = blk_cnt[hid];
+        // A block without particles (a "+1" neighbour only) has an all-zero tile:
+        // nothing is written and the grid update skips it.
+        if (nb == 0) continue;
         for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 192) tile[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         int bx, by, bz;
         uint32_t scene;
         unpack_key(P.batched, hdr_key[hid], &bx, &by, &bz, &scene);
-        const uint32_t b0 = blk_start[hid], nb = blk_cnt[hid];
         const uint32_t cs = cell_start[(size_t)hid * NODES_PER_BLOCK + lane];
         const uint32_t ce = cs + cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane];
         float nx[3], ny[3];
@@ -790,7 +925,8 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
                                                      float4 *__restrict__ nodes, const uint32_t *__restrict__ nbr,
                                                      const uint32_t *__restrict__ hdr_key, Colliders cs,
                                                      const NodeColl *__restrict__ coll_prev, NodeColl *__restrict__ coll_cur,
-                                                     const uint32_t *__restrict__ counters, uint32_t mode,
+                                                     const uint32_t *__restrict__ counters,
+                                                     const uint32_t *__restrict__ blk_cnt, uint32_t mode,
                                                      uint32_t do_update) {
     const uint32_t B = counters[CTR_NUM_BLOCKS];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -799,13 +935,14 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
     for (uint32_t hid = blockIdx.x * 4 + wave; hid < B; hid += waves_per_grid) {
         float4 acc;
         if (mode == 0) {
-            acc = tiles[(size_t)hid * TILE_NODES + x + 6 * y + 36 * z];
+            // Tiles of blocks without particles were not written (all zero).
+            acc = blk_cnt[hid] ? tiles[(size_t)hid * TILE_NODES + x + 6 * y + 36 * z] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
             for (int d = 1; d < 8; d++) {
                 const int ox = ASSOC[d][0], oy = ASSOC[d][1], oz = ASSOC[d][2];
                 if ((ox == 0 || x <= 1) && (oy == 0 || y <= 1) && (oz == 0 || z <= 1)) {
                     uint32_t src = nbr[(size_t)hid * NBR_STRIDE + NBR_MINUS + d - 1];
-                    if (src != SLOSH_NONE) {
+                    if (src != SLOSH_NONE && blk_cnt[src] != 0u) {
                         float4 t = tiles[(size_t)src * TILE_NODES + (x + 4 * ox) + 6 * (y + 4 * oy) + 36 * (z + 4 * oz)];
                         acc.x += t.x;
                         acc.y += t.y;
@@ -944,8 +1081,8 @@ __device__ inline void load_halo(float4 *__restrict__ sh, const float4 *__restri
     }
 }
 
-template <uint32_t MODELS>
-__global__ __launch_bounds__(256) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
+template <uint32_t MODELS, int NT>
+__global__ __launch_bounds__(NT) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
                                                     float *__restrict__ fdt, const uint32_t *__restrict__ perm,
                                                     const uint32_t *__restrict__ blk_start,
                                                     const uint32_t *__restrict__ blk_cnt,
@@ -957,9 +1094,10 @@ __global__ __launch_bounds__(256) void k_g2p_update(StepParams P, PBuf in, PBuf 
     if (blockIdx.x == 0 && threadIdx.x < bodies.count) integrate_body(P, bodies, threadIdx.x);
     const uint32_t B = counters[CTR_NUM_BLOCKS];
     for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
+        const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
+        if (cnt == 0) continue;
         load_halo(sh, nodes, nbr, hid);
         __syncthreads();
-        const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
         // Every lane of the workgroup runs the same number of iterations so the
         // wave-level run detection of the next table's insertion sees all lanes.
         for (uint32_t jb = 0; jb < cnt; jb += blockDim.x) {
@@ -1031,9 +1169,10 @@ __global__ __launch_bounds__(256) void k_g2p(StepParams P, PBuf buf, uint32_t pc
     __shared__ float4 sh[TILE_NODES];
     const uint32_t B = counters[CTR_NUM_BLOCKS];
     for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
+        const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
+        if (cnt == 0) continue;
         load_halo(sh, nodes, nbr, hid);
         __syncthreads();
-        const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
         for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
             const uint32_t p = perm[s0 + j];
             const float4 ps = buf.pos[p];
@@ -1152,7 +1291,11 @@ static inline PBuf to_pbuf(const ParticleBuffers &b) {
     return p;
 }
 
-uint32_t block_grid_size(uint32_t cap, uint32_t per_wg) { return min_t(div_up_t(cap, per_wg), 8192); }
+// Workgroups of a per-block kernel: one block (per_wg blocks) per workgroup up to
+// the tuned persistent width; workgroups loop over the device-side block count.
+static uint32_t block_grid(const StepParams &P, uint32_t per_wg) {
+    return min_t(div_up_t(P.cap, per_wg), P.block_wgs ? P.block_wgs : 8192u);
+}
 
 hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t pcap, const TransferBuffers &t,
                       const NextTableBuffers &next, const PassEvents *ev, hipStream_t stream) {
@@ -1161,9 +1304,10 @@ hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t p
     nt.vals = next.vals;
     nt.counters = next.counters;
     nt.prime = next.prime;
-    launch_kernel(ev, k_p2g<512>, dim3(block_grid_size(P.cap, 1)), dim3(192), 0, stream, P, nt,
-                  (const float4 *)pb.pos, (const float4 *)pb.vm, (const float *)pb.C, (const float *)t.fdt, pcap,
-                  t.perm, t.cell_cnt, t.cell_start, t.blk_start, t.blk_cnt, t.hdr_key, t.tiles, t.counters);
+    auto kern = P.p2g_chunk == 256u ? k_p2g<256> : (P.p2g_chunk == 1024u ? k_p2g<1024> : k_p2g<512>);
+    launch_kernel(ev, kern, dim3(block_grid(P, 1)), dim3(192), 0, stream, P, nt, (const float4 *)pb.pos,
+                  (const float4 *)pb.vm, (const float *)pb.C, (const float *)t.fdt, pcap, t.perm, t.cell_cnt,
+                  t.cell_start, t.blk_start, t.blk_cnt, t.hdr_key, t.tiles, t.counters);
     return hipGetLastError();
 }
 
@@ -1174,9 +1318,9 @@ hipError_t launch_grid_update(const StepParams &P, const TransferBuffers &t, con
     cs.poses = c.poses;
     cs.mats = c.mats;
     cs.count = c.count;
-    launch_kernel(ev, k_grid_update, dim3(block_grid_size(P.cap, 4)), dim3(256), 0, stream, P,
+    launch_kernel(ev, k_grid_update, dim3(block_grid(P, 4)), dim3(256), 0, stream, P,
                   (const float4 *)t.tiles, t.nodes, t.nbr, t.hdr_key, cs, (const NodeColl *)t.coll_prev,
-                  (NodeColl *)t.coll_cur, t.counters, mode, do_update);
+                  (NodeColl *)t.coll_cur, t.counters, t.blk_cnt, mode, do_update);
     return hipGetLastError();
 }
 
@@ -1201,20 +1345,19 @@ hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, con
     st.model = s.model;
     st.dp0 = s.dp0;
     st.dp1 = s.dp1;
-    if (P.tag_mask == 2u)  // neo-Hookean only
-        launch_kernel(ev, k_g2p_update<2u>, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(in),
-                      to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes,
-                      t.counters, nt, b);
-    else
-        launch_kernel(ev, k_g2p_update<15u>, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(in),
-                      to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes,
-                      t.counters, nt, b);
+    const bool nh_only = P.tag_mask == 2u;  // neo-Hookean only: no SVD code in the kernel
+    const bool narrow = P.g2p_threads == 128u;
+    auto kern = nh_only ? (narrow ? k_g2p_update<2u, 128> : k_g2p_update<2u, 256>)
+                        : (narrow ? k_g2p_update<15u, 128> : k_g2p_update<15u, 256>);
+    launch_kernel(ev, kern, dim3(block_grid(P, 1)), dim3(narrow ? 128 : 256), 0, stream, P, to_pbuf(in),
+                  to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes,
+                  t.counters, nt, b);
     return hipGetLastError();
 }
 
 hipError_t launch_g2p(const StepParams &P, const ParticleBuffers &b, uint32_t pcap, const TransferBuffers &t,
                       const PassEvents *ev, hipStream_t stream) {
-    launch_kernel(ev, k_g2p, dim3(block_grid_size(P.cap, 1)), dim3(256), 0, stream, P, to_pbuf(b), pcap, t.perm,
+    launch_kernel(ev, k_g2p, dim3(block_grid(P, 1)), dim3(256), 0, stream, P, to_pbuf(b), pcap, t.perm,
                   t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes, t.counters);
     return hipGetLastError();
 }

@@ -36,6 +36,7 @@ struct GridBuffers {
     uint32_t *keys, *vals, *prev_keys, *prev_vals;
     uint32_t *hdr_key, *nbr, *cell_cnt, *cell_start, *blk_start, *blk_cnt;
     uint32_t *pslot, *prank, *perm, *counters;
+    uint32_t *tile_sums;  // ceil(cap / 4096) words
 };
 hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n, float h, uint32_t batched,
                              uint32_t cap, bool primed, const PassEvents *ev, hipStream_t stream);
@@ -149,6 +150,12 @@ struct PBufs {
     uint32_t *ids = nullptr;
 };
 
+struct HostRun {
+    uint32_t scene = 0;
+    std::vector<slosh_particle_model> models;
+    std::vector<slosh_particle_properties> props;
+};
+
 struct slosh_data {
     slosh_pipeline *pipeline = nullptr;
     size_t n = 0, pcap = 0;
@@ -178,13 +185,19 @@ struct slosh_data {
     uint32_t *scan_tmp = nullptr;  // scan tile sums + total
     uint8_t *dead = nullptr;       // per-scene removal flags [SLOSH_MAX_SCENES]
     std::vector<uint8_t> dead_host;  // source of the last flag upload (outlives the async copy)
-    std::vector<slosh_particle_model> models_host;      // by original index
-    std::vector<slosh_particle_properties> props_host;  // by original index
-    std::vector<uint32_t> scene_host;                   // by original index
+    slosh_tuning tuning{512u, 256u, 8192u};
+    std::vector<slosh_boundary_condition> mats_host;
+    // Creation-time records in original-id order, as runs of consecutive ids
+    // with one scene id (an appended scene is one run): removing a scene drops
+    // runs instead of rewriting per-particle host arrays.
+    std::vector<HostRun> runs;
+    uint64_t tag_count[5] = {0, 0, 0, 0, 0};  // particles per material tag (4: other)
+    int uni_state = 0;                        // 0: no particles, 1: uniform (u_* valid), 2: mixed
     // grid
     uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
     uint32_t *hdr_key = nullptr, *nbr = nullptr, *cell_cnt = nullptr, *cell_start = nullptr;
     uint32_t *blk_start = nullptr, *blk_cnt = nullptr, *pslot = nullptr, *prank = nullptr, *perm = nullptr;
+    uint32_t *blk_tile_sums = nullptr;
     uint32_t *counters = nullptr;
     float4 *tiles = nullptr, *nodes = nullptr;
     float4 *coll[2] = {nullptr, nullptr};
@@ -309,32 +322,81 @@ static bool model_is_sand(const slosh_particle_model &m) { return m.words[0] == 
 static void static_records(const slosh_particle_properties &pr, const slosh_particle_model &m, float4 *prop,
                            float4 *model, float4 *dp0, float4 *dp1);
 
-// Sets d->uniform when every particle has the same static material record (the
-// kernels then take it from their arguments instead of gathering it).
-static void update_uniform(slosh_data *d) {
-    d->uniform = 0;
-    const size_t n = d->models_host.size();
-    if (n == 0) return;
-    float4 p0, m0, a0, b0;
-    static_records(d->props_host[0], d->models_host[0], &p0, &m0, &a0, &b0);
-    for (size_t i = 1; i < n; i++) {
-        float4 p, m, a, b;
-        static_records(d->props_host[i], d->models_host[i], &p, &m, &a, &b);
-        if (std::memcmp(&p, &p0, 12) != 0 || std::memcmp(&m, &m0, 16) != 0 || std::memcmp(&a, &a0, 16) != 0 ||
-            std::memcmp(&b, &b0, 8) != 0)
-            return;
+static int tag_slot(const slosh_particle_model &m) { return m.words[0] < 4u ? (int)m.words[0] : 4; }
+
+// d->uniform is set while every particle has the same static material record
+// (the kernels then take it from their arguments instead of gathering it);
+// tracked incrementally as particles are added, reset when the data empties.
+static void note_records(slosh_data *d, const slosh_particle_model *m, const slosh_particle_properties *pr, size_t n,
+                         int sign) {
+    for (size_t i = 0; i < n; i++) d->tag_count[tag_slot(m[i])] += (uint64_t)(int64_t)sign;
+    uint32_t mask = 0;
+    for (int t = 0; t < 5; t++)
+        if (d->tag_count[t]) mask |= 1u << t;
+    d->tag_mask = mask;
+    if (sign > 0) {
+        for (size_t i = 0; i < n && d->uni_state != 2; i++) {
+            float4 p, mm, a, b;
+            static_records(pr[i], m[i], &p, &mm, &a, &b);
+            if (d->uni_state == 0) {
+                d->u_prop = p;
+                d->u_model = mm;
+                d->u_dp0 = a;
+                d->u_dp1 = b;
+                d->uni_state = 1;
+            } else if (std::memcmp(&p, &d->u_prop, 12) != 0 || std::memcmp(&mm, &d->u_model, 16) != 0 ||
+                       std::memcmp(&a, &d->u_dp0, 16) != 0 || std::memcmp(&b, &d->u_dp1, 8) != 0) {
+                d->uni_state = 2;
+            }
+        }
     }
-    d->uniform = 1;
-    d->u_prop = p0;
-    d->u_model = m0;
-    d->u_dp0 = a0;
-    d->u_dp1 = b0;
+    if (d->n == 0 && sign < 0) d->uni_state = 0;
+    d->uniform = d->uni_state == 1 ? 1u : 0u;
 }
 
-static uint32_t tag_mask_of(const std::vector<slosh_particle_model> &models) {
-    uint32_t mask = 0;
-    for (const auto &m : models) mask |= m.words[0] < 4u ? (1u << m.words[0]) : 16u;
-    return mask;
+// Appends records to the host runs (split where the scene id changes).
+static void host_append(slosh_data *d, const slosh_particles *p) {
+    size_t i = 0;
+    while (i < p->len) {
+        const uint32_t sc = p->scene_ids ? p->scene_ids[i] : 0u;
+        size_t j = i + 1;
+        while (j < p->len && (p->scene_ids ? p->scene_ids[j] : 0u) == sc) j++;
+        if (d->runs.empty() || d->runs.back().scene != sc) {
+            d->runs.emplace_back();
+            d->runs.back().scene = sc;
+        }
+        HostRun &r = d->runs.back();
+        r.models.insert(r.models.end(), p->models + i, p->models + j);
+        r.props.insert(r.props.end(), p->properties + i, p->properties + j);
+        i = j;
+    }
+}
+
+static std::vector<slosh_particle_model> flat_models(const slosh_data *d) {
+    std::vector<slosh_particle_model> v;
+    v.reserve(d->n);
+    for (const auto &r : d->runs) v.insert(v.end(), r.models.begin(), r.models.end());
+    return v;
+}
+static std::vector<slosh_particle_properties> flat_props(const slosh_data *d) {
+    std::vector<slosh_particle_properties> v;
+    v.reserve(d->n);
+    for (const auto &r : d->runs) v.insert(v.end(), r.props.begin(), r.props.end());
+    return v;
+}
+static std::vector<uint32_t> flat_scenes(const slosh_data *d) {
+    std::vector<uint32_t> v;
+    v.reserve(d->n);
+    for (const auto &r : d->runs) v.insert(v.end(), r.models.size(), r.scene);
+    return v;
+}
+// Writes models (original-id order) back into the runs.
+static void set_models(slosh_data *d, const std::vector<slosh_particle_model> &m) {
+    size_t o = 0;
+    for (auto &r : d->runs) {
+        std::copy(m.begin() + o, m.begin() + o + r.models.size(), r.models.begin());
+        o += r.models.size();
+    }
 }
 
 // Host-side staging of particles in the internal layout (SoA, slot order).
@@ -497,6 +559,9 @@ static StepParams make_params(const slosh_data *d) {
     P.svd_f64 = d->svd_f64;
     P.tag_mask = d->tag_mask;
     P.uniform = d->uniform;
+    P.p2g_chunk = d->tuning.p2g_chunk;
+    P.g2p_threads = d->tuning.g2p_threads;
+    P.block_wgs = d->tuning.block_wgs;
     P.u_prop = d->u_prop;
     P.u_model = d->u_model;
     P.u_dp0 = d->u_dp0;
@@ -580,6 +645,7 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     g.prank = d->prank;
     g.perm = d->perm;
     g.counters = ctr_set(d, cur_g);
+    g.tile_sums = d->blk_tile_sums;
 
     // The fused path (no hooks, built-in P2G and particle update) prepares the
     // next substep's table inside this substep's kernels.
@@ -722,6 +788,7 @@ static void free_data(slosh_data *d) {
     dfree(d->cell_start);
     dfree(d->blk_start);
     dfree(d->blk_cnt);
+    dfree(d->blk_tile_sums);
     dfree(d->counters);
     dfree(d->tiles);
     dfree(d->nodes);
@@ -811,12 +878,8 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     d->has_sand = (any_sand || d->batched) ? 1u : 0u;
     d->has_force = any_force ? 1u : 0u;
     TRY(alloc_particle_buffers(d, d->pcap, d->has_sand != 0));
-    d->models_host.assign(particles->models, particles->models + d->n);
-    d->tag_mask = tag_mask_of(d->models_host);
-    d->props_host.assign(particles->properties, particles->properties + d->n);
-    d->scene_host.resize(d->n);
-    for (size_t i = 0; i < d->n; i++) d->scene_host[i] = particles->scene_ids ? particles->scene_ids[i] : 0u;
-    update_uniform(d);
+    host_append(d, particles);
+    note_records(d, particles->models, particles->properties, particles->len, 1);
     HIP_TRY(hipStreamCreateWithFlags(&d->capture_stream, hipStreamNonBlocking));
     TRY(upload_staging(d, s, 0, 0, 0, d->capture_stream));
     const size_t cap = d->cap;
@@ -832,6 +895,7 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     TRY(dalloc(&d->cell_start, cap * NODES_PER_BLOCK));
     TRY(dalloc(&d->blk_start, cap));
     TRY(dalloc(&d->blk_cnt, cap));
+    TRY(dalloc(&d->blk_tile_sums, (cap + 4095) / 4096 + 1));
     TRY(dalloc(&d->counters, 16));
     HIP_TRY(hipMemset(d->counters, 0, 64));
     TRY(dalloc(&d->dead, SLOSH_MAX_SCENES + 1));
@@ -839,6 +903,7 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     TRY(dalloc(&d->nodes, cap * NODES_PER_BLOCK));
     d->nbodies = (uint32_t)num_bodies;
     d->bodies_host.assign(bodies, bodies + num_bodies);
+    d->mats_host.assign(body_materials, body_materials + num_bodies);
     bool moving = false;
     for (size_t i = 0; i < num_bodies; i++) moving = moving || body_may_move(bodies[i]);
     d->coll_cache = moving ? 1u : 0u;
@@ -1083,9 +1148,10 @@ int slosh_read_models(slosh_data *d, slosh_stream stream, slosh_particle_model *
     Snapshot s;
     TRY(download_current(d, (hipStream_t)stream, s));
     const size_t n = d->n;
+    const std::vector<slosh_particle_model> models = flat_models(d);
     for (size_t i = 0; i < n; i++) {
         uint32_t id = s.ids[i];
-        out[id] = d->models_host[id];
+        out[id] = models[id];
         if (model_is_sand(out[id]))
             for (int c = 0; c < 3; c++) out[id].words[1 + c] = f2u(s.dps[(size_t)c * n + i]);
     }
@@ -1095,14 +1161,22 @@ int slosh_read_models(slosh_data *d, slosh_stream stream, slosh_particle_model *
 int slosh_read_properties(slosh_data *d, slosh_stream stream, slosh_particle_properties *out) {
     (void)stream;
     if (!d || (!out && d->n)) return fail(SLOSH_E_INVALID, "null argument");
-    std::memcpy(out, d->props_host.data(), sizeof(slosh_particle_properties) * d->n);
+    size_t o = 0;
+    for (const auto &r : d->runs) {
+        std::copy(r.props.begin(), r.props.end(), out + o);
+        o += r.props.size();
+    }
     return SLOSH_OK;
 }
 
 int slosh_read_scene_ids(slosh_data *d, slosh_stream stream, uint32_t *out) {
     (void)stream;
     if (!d || (!out && d->n)) return fail(SLOSH_E_INVALID, "null argument");
-    std::memcpy(out, d->scene_host.data(), 4 * d->n);
+    size_t o = 0;
+    for (const auto &r : d->runs) {
+        std::fill(out + o, out + o + r.models.size(), r.scene);
+        o += r.models.size();
+    }
     return SLOSH_OK;
 }
 
@@ -1228,23 +1302,45 @@ static int rebuild(slosh_data *d, hipStream_t st, size_t new_cap, const Staging 
 }
 
 // Current state as a Staging in original order (static parts from host copies).
-static int gather_all(slosh_data *d, hipStream_t st, Staging &all, bool *any_force) {
+// Current state in the reference layouts, original order.
+struct RefArrays {
+    std::vector<slosh_kinematics> kin;
+    std::vector<slosh_matrix> F;
+    std::vector<float> pos;
+    std::vector<slosh_particle_model> models;
+    std::vector<slosh_particle_properties> props;
+    std::vector<uint32_t> scenes;
+    slosh_particles view(bool batched) const {
+        slosh_particles p;
+        p.len = kin.size();
+        p.positions = pos.data();
+        p.kinematics = kin.data();
+        p.def_grad = F.data();
+        p.properties = props.data();
+        p.models = models.data();
+        p.scene_ids = batched ? scenes.data() : nullptr;
+        return p;
+    }
+};
+
+static int gather_reference(slosh_data *d, hipStream_t st, RefArrays &r) {
     Snapshot s;
     TRY(download_current(d, st, s));
     const size_t n = d->n;
-    std::vector<slosh_kinematics> kin(n);
-    std::vector<slosh_matrix> F(n);
-    std::vector<float> pos(4 * n);
-    std::vector<slosh_particle_model> models(n);
+    r.kin.resize(n);
+    r.F.resize(n);
+    r.pos.assign(4 * n, 0.0f);
+    r.models.resize(n);
+    const std::vector<slosh_particle_model> old_models = flat_models(d);
     for (size_t i = 0; i < n; i++) {
         uint32_t id = s.ids[i];
-        for (int c = 0; c < 3; c++) pos[4 * id + c] = s.pos[4 * i + c];
-        slosh_kinematics &k = kin[id];
+        for (int c = 0; c < 3; c++) r.pos[4 * id + c] = s.pos[4 * i + c];
+        slosh_kinematics &k = r.kin[id];
         std::memset(&k, 0, sizeof k);
-        for (int r = 0; r < 3; r++)
+        for (int a = 0; a < 3; a++)
             for (int c = 0; c < 3; c++) {
-                k.affine.m[r][c] = s.C[(size_t)(r * 3 + c) * n + i];
-                F[id].m[r][c] = s.F[(size_t)(r * 3 + c) * n + i];
+                k.affine.m[a][c] = s.C[(size_t)(a * 3 + c) * n + i];
+                r.F[id].m[a][c] = s.F[(size_t)(a * 3 + c) * n + i];
             }
         for (int c = 0; c < 3; c++) {
             k.velocity[c] = s.vm[4 * i + c];
@@ -1253,19 +1349,21 @@ static int gather_all(slosh_data *d, hipStream_t st, Staging &all, bool *any_for
         k.vel_grad_det = s.vgd[i];
         k.mass = s.vm[4 * i + 3];
         k.enabled = f2u(s.pos[4 * i + 3]) & 1u;
-        models[id] = d->models_host[id];
-        if (model_is_sand(models[id]))
-            for (int c = 0; c < 3; c++) models[id].words[1 + c] = f2u(s.dps[(size_t)c * n + i]);
+        r.models[id] = old_models[id];
+        if (model_is_sand(r.models[id]))
+            for (int c = 0; c < 3; c++) r.models[id].words[1 + c] = f2u(s.dps[(size_t)c * n + i]);
     }
-    d->models_host = models;
-    slosh_particles p;
-    p.len = n;
-    p.positions = pos.data();
-    p.kinematics = kin.data();
-    p.def_grad = F.data();
-    p.properties = d->props_host.data();
-    p.models = models.data();
-    p.scene_ids = d->batched ? d->scene_host.data() : nullptr;
+    set_models(d, r.models);
+    r.props = flat_props(d);
+    r.scenes = flat_scenes(d);
+    return SLOSH_OK;
+}
+
+// Current state as a Staging in original order.
+static int gather_all(slosh_data *d, hipStream_t st, Staging &all, bool *any_force) {
+    RefArrays r;
+    TRY(gather_reference(d, st, r));
+    const slosh_particles p = r.view(d->batched != 0);
     bool any_sand = false;
     return stage_particles(&p, 0, all, any_force, &any_sand);
 }
@@ -1309,40 +1407,35 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
     bool any_force = false, any_sand = false;
     Staging s;
     TRY(stage_particles(particles, (uint32_t)d->n, s, &any_force, &any_sand));
+    // Host staging above overlaps substeps still running on `stream`; the graphs
+    // they replay are released only once they are done.
+    HIP_TRY(hipStreamSynchronize(st));
     invalidate_graphs(d);
-    d->models_host.insert(d->models_host.end(), particles->models, particles->models + particles->len);
-    d->props_host.insert(d->props_host.end(), particles->properties, particles->properties + particles->len);
-    for (size_t i = 0; i < particles->len; i++)
-        d->scene_host.push_back(particles->scene_ids ? particles->scene_ids[i] : 0u);
     const bool need_sand = any_sand && !d->has_sand;
     if (d->n + particles->len <= d->pcap && !need_sand) {
         TRY(upload_staging(d, s, d->cur_p, d->n, d->n, st));
         d->n += particles->len;
     } else {
-        // GpuTensor::append reallocates; we rebuild in original order with headroom.
+        // GpuTensor::append reallocates; we rebuild in original order with headroom
+        // (rebuild sets d->n to the combined count).
         Staging all;
         bool f2 = false;
-        // models_host already holds the appended records: gather the old part only.
-        const size_t old_n = d->n;
-        std::vector<slosh_particle_model> saved(d->models_host.begin() + old_n, d->models_host.end());
-        d->models_host.resize(old_n);
         TRY(gather_all(d, st, all, &f2));
-        d->models_host.insert(d->models_host.end(), saved.begin(), saved.end());
         concat_staging(all, s);
         any_force = any_force || f2;
         TRY(rebuild(d, st, (d->n + particles->len) * 3 / 2 + 1024, all, any_sand));
     }
+    host_append(d, particles);
+    note_records(d, particles->models, particles->properties, particles->len, 1);
     if (any_force) d->has_force = 1;
-    d->tag_mask = tag_mask_of(d->models_host);
-    update_uniform(d);
     return SLOSH_OK;
 }
 
 // Device-side removal (mode 0: the scenes flagged in dead_host; mode 1: ids in
 // [lo, hi)). Kept particles are compacted into the other particle buffer in slot
-// order and their original ids renumbered in order; the host mirrors follow.
+// order and their original ids renumbered in order; the host runs follow.
 static int device_remove(slosh_data *d, hipStream_t st, uint32_t mode, const std::vector<uint8_t> &dead_host,
-                         size_t lo, size_t hi, const std::vector<uint8_t> &keep_by_id) {
+                         size_t lo, size_t hi) {
     const size_t n = d->n;
     invalidate_graphs(d);
     if (mode == 0) {
@@ -1384,20 +1477,41 @@ static int device_remove(slosh_data *d, hipStream_t st, uint32_t mode, const std
     std::swap(d->s_model, d->s2_model);
     std::swap(d->s_dp0, d->s2_dp0);
     std::swap(d->s_dp1, d->s2_dp1);
-    size_t m = 0;
-    for (size_t i = 0; i < n; i++) {
-        if (!keep_by_id[i]) continue;
-        d->models_host[m] = d->models_host[i];
-        d->props_host[m] = d->props_host[i];
-        d->scene_host[m] = d->scene_host[i];
-        m++;
+    // Host runs: drop removed runs (mode 0) or cut [lo, hi) out of them (mode 1).
+    std::vector<HostRun> kept;
+    size_t off = 0, m = 0;
+    for (auto &r : d->runs) {
+        const size_t len = r.models.size();
+        if (mode == 0) {
+            if (dead_host[r.scene]) {
+                note_records(d, r.models.data(), r.props.data(), len, -1);
+            } else {
+                m += len;
+                kept.push_back(std::move(r));
+            }
+        } else {
+            const size_t a0 = std::max(lo, off), a1 = std::min(hi, off + len);
+            if (a0 < a1) {
+                note_records(d, r.models.data() + (a0 - off), r.props.data() + (a0 - off), a1 - a0, -1);
+                r.models.erase(r.models.begin() + (a0 - off), r.models.begin() + (a1 - off));
+                r.props.erase(r.props.begin() + (a0 - off), r.props.begin() + (a1 - off));
+            }
+            if (!r.models.empty()) {
+                m += r.models.size();
+                if (!kept.empty() && kept.back().scene == r.scene) {
+                    HostRun &k = kept.back();
+                    k.models.insert(k.models.end(), r.models.begin(), r.models.end());
+                    k.props.insert(k.props.end(), r.props.begin(), r.props.end());
+                } else {
+                    kept.push_back(std::move(r));
+                }
+            }
+        }
+        off += len;
     }
-    d->models_host.resize(m);
-    d->props_host.resize(m);
-    d->scene_host.resize(m);
+    d->runs = std::move(kept);
     d->n = m;
-    d->tag_mask = tag_mask_of(d->models_host);
-    update_uniform(d);
+    note_records(d, nullptr, nullptr, 0, -1);  // refresh the uniform state once the count is known
     return SLOSH_OK;
 }
 
@@ -1405,9 +1519,7 @@ int slosh_remove_particles(slosh_data *d, slosh_stream stream, size_t start, siz
     if (!d) return fail(SLOSH_E_INVALID, "null data");
     if (start > end || end > d->n) return fail(SLOSH_E_INVALID, "removal range out of bounds");
     if (start == end) return SLOSH_OK;
-    std::vector<uint8_t> keep(d->n, 1);
-    for (size_t i = start; i < end; i++) keep[i] = 0;
-    return device_remove(d, (hipStream_t)stream, 1, {}, start, end, keep);
+    return device_remove(d, (hipStream_t)stream, 1, {}, start, end);
 }
 
 int slosh_remove_scenes(slosh_data *d, slosh_stream stream, const uint32_t *scene_ids, size_t count,
@@ -1418,16 +1530,12 @@ int slosh_remove_scenes(slosh_data *d, slosh_stream stream, const uint32_t *scen
         if (scene_ids[k] >= SLOSH_MAX_SCENES) return fail(SLOSH_E_INVALID, "scene id out of range");
         dead[scene_ids[k]] = 1;
     }
-    std::vector<uint8_t> keep(d->n, 1);
     size_t cnt = 0;
-    for (size_t i = 0; i < d->n; i++)
-        if (dead[d->scene_host[i]]) {
-            keep[i] = 0;
-            cnt++;
-        }
+    for (const auto &r : d->runs)
+        if (dead[r.scene]) cnt += r.models.size();
     if (removed) *removed = cnt;
     if (cnt == 0) return SLOSH_OK;
-    return device_remove(d, (hipStream_t)stream, 0, dead, 0, 0, keep);
+    return device_remove(d, (hipStream_t)stream, 0, dead, 0, 0);
 }
 
 int slosh_remove_scene(slosh_data *d, slosh_stream stream, uint32_t scene_id, size_t *removed) {
@@ -1437,7 +1545,8 @@ int slosh_remove_scene(slosh_data *d, slosh_stream stream, uint32_t scene_id, si
 size_t slosh_scene_size(const slosh_data *d, uint32_t scene_id) {
     if (!d) return 0;
     size_t c = 0;
-    for (uint32_t s : d->scene_host) c += s == scene_id;
+    for (const auto &r : d->runs)
+        if (r.scene == scene_id) c += r.models.size();
     return c;
 }
 
@@ -1507,6 +1616,115 @@ int slosh_read_scene(slosh_data *d, slosh_stream stream, uint32_t scene_id, floa
                 for (int c = 0; c < 3; c++) mm.m[r][c] = F[(size_t)(r * 3 + c) * m + j];
         }
     }
+    return SLOSH_OK;
+}
+
+// ---- launch-shape tuning ------------------------------------------------------
+static bool tuning_valid(const slosh_tuning &t) {
+    return (t.p2g_chunk == 256u || t.p2g_chunk == 512u || t.p2g_chunk == 1024u) &&
+           (t.g2p_threads == 128u || t.g2p_threads == 256u) && t.block_wgs >= 64u && t.block_wgs <= 65536u;
+}
+
+int slosh_set_tuning(slosh_data *d, const slosh_tuning *t) {
+    if (!d || !t) return fail(SLOSH_E_INVALID, "null argument");
+    if (!tuning_valid(*t)) return fail(SLOSH_E_INVALID, "unsupported tuning values");
+    d->tuning = *t;
+    invalidate_graphs(d);
+    return SLOSH_OK;
+}
+
+int slosh_get_tuning(const slosh_data *d, slosh_tuning *out) {
+    if (!d || !out) return fail(SLOSH_E_INVALID, "null argument");
+    *out = d->tuning;
+    return SLOSH_OK;
+}
+
+// Times `steps` graph-replayed substeps of `c` under tuning `t` (best of two).
+static int time_tuning(slosh_pipeline *pl, slosh_data *c, hipStream_t st, const slosh_tuning &t, uint32_t steps,
+                       float *ms) {
+    c->tuning = t;
+    invalidate_graphs(c);
+    TRY(slosh_launch_steps_graph(pl, c, st, 2));  // capture under this shape
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int rc = SLOSH_OK;
+    for (int rep = 0; rep < 2 && rc == SLOSH_OK; rep++) {
+        (void)hipEventRecord(e0, st);
+        rc = slosh_launch_steps_graph(pl, c, st, steps);
+        (void)hipEventRecord(e1, st);
+        if (rc == SLOSH_OK && hipEventSynchronize(e1) == hipSuccess) {
+            float v = 0.0f;
+            if (hipEventElapsedTime(&v, e0, e1) == hipSuccess) best = std::min(best, v);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    TRY(rc);
+    TRY(slosh_check_errors(c, (slosh_stream)st));
+    *ms = best;
+    return SLOSH_OK;
+}
+
+int slosh_autotune(slosh_pipeline *pl, slosh_data *d, slosh_stream stream, uint32_t trial_steps, slosh_tuning *out) {
+    if (!pl || !d) return fail(SLOSH_E_INVALID, "null pipeline or data");
+    hipStream_t st = (hipStream_t)stream;
+    if (d->n == 0) {
+        if (out) *out = d->tuning;
+        return SLOSH_OK;
+    }
+    const uint32_t steps = std::max<uint32_t>(2u, trial_steps + (trial_steps & 1u));
+    // Trials run on a private copy so that `d` does not advance.
+    RefArrays r;
+    TRY(gather_reference(d, st, r));
+    const slosh_particles p = r.view(d->batched != 0);
+    slosh_data *c = nullptr;
+    TRY(slosh_data_create(pl, &d->params, &p, d->pcap, d->bodies_host.data(), d->nbodies, d->mats_host.data(), d->h,
+                          d->cap, &c));
+    c->svd_f64 = d->svd_f64;
+    slosh_tuning best = d->tuning;
+    float best_ms = 0.0f;
+    int rc = run_step(pl, c, st, nullptr, nullptr, nullptr);
+    if (rc == SLOSH_OK) rc = time_tuning(pl, c, st, best, steps, &best_ms);
+    // Coordinate search: P2G staging chunk, G2P workgroup size, persistent width.
+    const uint32_t chunks[] = {256u, 512u, 1024u};
+    const uint32_t threads[] = {128u, 256u};
+    const uint32_t widths[] = {2048u, 4096u, 8192u, 16384u, 32768u};
+    auto consider = [&](slosh_tuning t) {
+        if (rc != SLOSH_OK || std::memcmp(&t, &best, sizeof t) == 0) return;
+        float ms = 0.0f;
+        rc = time_tuning(pl, c, st, t, steps, &ms);
+        if (rc == SLOSH_OK && ms < best_ms) {
+            best = t;
+            best_ms = ms;
+        }
+    };
+    for (uint32_t v : chunks) {
+        slosh_tuning t = best;
+        t.p2g_chunk = v;
+        consider(t);
+    }
+    for (uint32_t v : threads) {
+        slosh_tuning t = best;
+        t.g2p_threads = v;
+        consider(t);
+    }
+    for (uint32_t v : widths) {
+        if (v > 2 * d->cap) continue;  // wider than the block capacity adds nothing
+        slosh_tuning t = best;
+        t.block_wgs = v;
+        consider(t);
+    }
+    std::string msg = g_last_error;
+    slosh_data_destroy(c);
+    if (rc != SLOSH_OK) {
+        g_last_error = msg;
+        return rc;
+    }
+    d->tuning = best;
+    invalidate_graphs(d);
+    if (out) *out = best;
     return SLOSH_OK;
 }
 

@@ -344,6 +344,25 @@ class MpmData:
         _lib.check(_lib.load().slosh_remove_scene(self.handle, self._s(stream), int(scene_id), C.byref(r)))
         return r.value
 
+    @property
+    def tuning(self):
+        t = _lib.Tuning()
+        _lib.check(_lib.load().slosh_get_tuning(self.handle, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = self.tuning
+        t.update(kw)
+        _lib.check(_lib.load().slosh_set_tuning(self.handle, C.byref(_lib.Tuning(**t))))
+
+    def autotune(self, pipeline, stream=None, trial_steps=10):
+        """Picks the fastest launch shape on a private copy of this state (the
+        state itself does not advance); returns it as a dict."""
+        t = _lib.Tuning()
+        _lib.check(_lib.load().slosh_autotune(pipeline.handle, self.handle, self._s(stream), int(trial_steps),
+                                              C.byref(t)))
+        return t.as_dict()
+
     def remove_scenes(self, scene_ids, stream=None):
         ids = np.ascontiguousarray(scene_ids, dtype=np.uint32)
         r = C.c_size_t()
@@ -353,17 +372,19 @@ class MpmData:
     def scene_size(self, scene_id):
         return int(_lib.load().slosh_scene_size(self.handle, int(scene_id)))
 
-    def read_scene(self, scene_id, stream=None):
+    def read_scene(self, scene_id, stream=None, positions_only=False):
         """(positions [m, 4], kinematics [m], def_grad [m, 3, 4]) of one scene in creation
-        order, in the formats of positions() / kinematics() / def_grad()."""
+        order, in the formats of positions() / kinematics() / def_grad(); with
+        positions_only, just the positions."""
         m = self.scene_size(scene_id)
-        pos = np.zeros((m, 4), np.float32)
-        kin = np.zeros(m, L.KINEMATICS)
-        f = np.zeros((m, 3, 4), np.float32)
+        pos = np.empty((m, 4), np.float32)
+        kin = None if positions_only else np.zeros(m, L.KINEMATICS)
+        f = None if positions_only else np.zeros((m, 3, 4), np.float32)
         cnt = C.c_size_t()
-        _lib.check(_lib.load().slosh_read_scene(self.handle, self._s(stream), int(scene_id), _p(pos), _p(kin), _p(f),
-                                                m, C.byref(cnt)))
-        return pos, kin, f
+        _lib.check(_lib.load().slosh_read_scene(self.handle, self._s(stream), int(scene_id), _p(pos),
+                                                _p(kin) if kin is not None else None,
+                                                _p(f) if f is not None else None, m, C.byref(cnt)))
+        return pos if positions_only else (pos, kin, f)
 
     def write_kinematics(self, kin, stream=None):
         kin = np.ascontiguousarray(kin, dtype=L.KINEMATICS)

@@ -113,6 +113,7 @@ class SceneServer:
         self.substeps_run = 0
         self.particle_substeps = 0
         self.chunks = 0
+        self.seconds = {"admit": 0.0, "run": 0.0, "retire": 0.0, "idle": 0.0}
 
     # -- admission / retirement -------------------------------------------------
     def submit(self, req):
@@ -125,6 +126,7 @@ class SceneServer:
 
     def admit(self, now):
         """Admits queued requests (FIFO) while particles, grid slots and scene ids allow."""
+        t0 = time.perf_counter()
         admitted = []
         while self.queue and self.free_ids:
             req = self.queue[0]
@@ -143,6 +145,7 @@ class SceneServer:
             self.in_flight += req.n
             admitted.append(req)
         self.peak_in_flight = max(self.peak_in_flight, self.in_flight)
+        self.seconds["admit"] += time.perf_counter() - t0
         return admitted
 
     def retire(self, clock):
@@ -150,13 +153,14 @@ class SceneServer:
         done = [sid for sid, (_, left) in self.running.items() if left == 0]
         if not done:
             return []
+        t0 = time.perf_counter()
         out = []
         for sid in done:
             req = self.running[sid][0]
             if self.result_kind == "full":
                 req.result = self.data.read_scene(sid, self.stream)
             else:
-                req.result = self.data.read_scene(sid, self.stream)[0][:, :3].copy()
+                req.result = self.data.read_scene(sid, self.stream, positions_only=True)[:, :3]
         self.data.remove_scenes(done, self.stream)
         now = clock()
         for sid in done:
@@ -166,24 +170,38 @@ class SceneServer:
             self.reserved_blocks -= self._reservation(req)
             self.in_flight -= req.n
             out.append(req)
+        self.seconds["retire"] += time.perf_counter() - t0
         return out
 
     # -- stepping ---------------------------------------------------------------
-    def run_chunk(self, until=None):
-        """Runs min(substeps left of any running scene, max_chunk) substeps of the
-        batch, and no further than substep count `until` if given."""
+    def launch_chunk(self, until=None):
+        """Enqueues min(substeps left of any running scene, max_chunk) substeps of the
+        batch (no further than substep count `until` if given) without waiting;
+        returns the token finish_chunk takes."""
+        t0 = time.perf_counter()
         k = min(min(left for _, left in self.running.values()), self.max_chunk)
         if until is not None and until > self.substeps_run:
             k = min(k, int(until) - self.substeps_run)
         self.pipeline.launch_steps_graph(self.data, k, self.stream)
+        self.seconds["run"] += time.perf_counter() - t0
+        return k, list(self.running.values()), self.in_flight
+
+    def finish_chunk(self, token):
+        """Waits for a launched chunk and advances the scenes it ran."""
+        t0 = time.perf_counter()
+        k, entries, in_flight = token
         self.stream.synchronize()
         self.data.check_errors(self.stream)
-        for entry in self.running.values():
+        for entry in entries:
             entry[1] -= k
         self.substeps_run += k
-        self.particle_substeps += k * self.in_flight
+        self.particle_substeps += k * in_flight
         self.chunks += 1
+        self.seconds["run"] += time.perf_counter() - t0
         return k
+
+    def run_chunk(self, until=None):
+        return self.finish_chunk(self.launch_chunk(until))
 
     def serve(self, requests, realtime=True):
         """Serves `requests`. With realtime, a request becomes visible at its arrival
@@ -202,14 +220,25 @@ class SceneServer:
             self.admit(now)
             if not self.running:
                 if pending and realtime:
+                    t1 = time.perf_counter()
                     time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.05)))
+                    self.seconds["idle"] += time.perf_counter() - t1
                 elif pending:
                     self.substeps_run = max(self.substeps_run, int(np.ceil(pending[0].arrival)))
                 elif self.queue:
                     raise RuntimeError("queued request can never be admitted")
                 continue
             # Without a wall clock, stop at the next arrival so it joins on time.
-            self.run_chunk(None if realtime or not pending else np.ceil(pending[0].arrival))
+            token = self.launch_chunk(None if realtime or not pending else np.ceil(pending[0].arrival))
+            if realtime:
+                # While the GPU runs the chunk, stage the requests that arrived
+                # meanwhile (their particles go to free slots the chunk does not
+                # touch; they join from the next chunk).
+                now = clock()
+                while pending and pending[0].arrival <= now:
+                    self.submit(pending.popleft())
+                self.admit(now)
+            self.finish_chunk(token)
             finished += self.retire(clock)
         return finished
 

@@ -40,3 +40,9 @@ def state_errors(gpu, ref, x0):
         "enabled_diff": int(np.sum(gk["enabled"] != rk["enabled"])),
     }
     return out
+
+
+def tol_for(steps):
+    """Relative L2 bound after `steps` substeps: 1e-5 after one substep and 1e-3
+    after 100 (the parity gates), linear in between."""
+    return 1.0e-5 * min(max(int(steps), 1), 100)

@@ -12,7 +12,7 @@ import pytest
 from slosh_amd import pipeline as P
 from slosh_amd import scenes
 from oracle import oracle as O
-from tests.parity import assert_blocks_equal, state_errors
+from tests.parity import assert_blocks_equal, state_errors, tol_for
 
 pytestmark = pytest.mark.gpu
 
@@ -97,10 +97,11 @@ def test_device_remove_scenes_keeps_others_exact(gpu_pipeline):
         pl.launch_step(data, st)
         pl.launch_step(ref, st)
     data.check_errors(st)
+    # Equal up to summation order within cells (7 substeps).
     for sid in (0, 3):
         a, r = data.read_scene(sid, st), ref.read_scene(sid, st)
         errs = state_errors(a, r, r[0])
-        assert max(errs[k] for k in ("x", "v", "affine", "F")) <= 1e-5, errs
+        assert max(errs[k] for k in ("x", "v", "affine", "F")) <= tol_for(7), errs
 
 
 def test_shift_remove_on_device_then_step(gpu_pipeline):
@@ -132,4 +133,35 @@ def test_append_scene_to_running_batch(gpu_pipeline):
     orc.step(5)
     errs = state_errors(data.read_scene(40, st), (orc.positions(), orc.kinematics(), orc.def_grad()),
                         extra.particles["pos4"])
-    assert max(errs[k] for k in ("x", "v", "affine", "F")) <= 1e-4, errs
+    assert max(errs[k] for k in ("x", "v", "affine", "F")) <= tol_for(5), errs
+
+
+def test_autotune_keeps_state_and_results(gpu_pipeline):
+    """The autotuner times candidates on a private copy: the data does not advance,
+    and a substep under the chosen launch shape matches the default shape."""
+    pl, st = gpu_pipeline
+    b = small_batch()
+    a = P.MpmData.from_batch(pl, b)
+    ref = P.MpmData.from_batch(pl, b)
+    pl.launch_step(a, st)
+    pl.launch_step(ref, st)
+    before = a.positions(st)
+    t = a.autotune(pl, st, trial_steps=4)
+    assert t == a.tuning and t["p2g_chunk"] in (256, 512, 1024) and t["g2p_threads"] in (128, 256)
+    assert np.array_equal(a.positions(st), before)
+    steps = 1
+    for shape in ({"p2g_chunk": 256, "g2p_threads": 128, "block_wgs": 2048},
+                  {"p2g_chunk": 1024, "g2p_threads": 256, "block_wgs": 16384}):
+        a.set_tuning(**shape)
+        for _ in range(3):
+            pl.launch_step(a, st)
+            pl.launch_step(ref, st)
+        steps += 3
+        a.check_errors(st)
+        # Equal up to summation order within cells (particle order within a cell
+        # follows atomic ranks, which vary between runs).
+        for sid in b.ids:
+            errs = state_errors(a.read_scene(sid, st), ref.read_scene(sid, st), ref.read_scene(sid, st)[0])
+            assert max(errs[k] for k in ("x", "v", "affine", "F")) <= tol_for(steps), (shape, sid, errs)
+    with pytest.raises(Exception):
+        a.set_tuning(p2g_chunk=300)
