@@ -315,41 +315,6 @@ __device__ inline double det3d(const double *a) {
     return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
 }
 
-// Singular values of a, descending, with sig[2] carrying the sign of det(a) (the
-// convention of svd3), from the eigenvalues of a^T a by the trigonometric closed
-// form for symmetric 3x3 matrices (O. K. Smith, "Eigenvalues of a symmetric 3 x 3
-// matrix", Commun. ACM 4(4), 1961).
-__device__ inline void singular_values_f64(const double *a, double *sig) {
-    double c00 = a[0] * a[0] + a[3] * a[3] + a[6] * a[6];
-    double c11 = a[1] * a[1] + a[4] * a[4] + a[7] * a[7];
-    double c22 = a[2] * a[2] + a[5] * a[5] + a[8] * a[8];
-    double c01 = a[0] * a[1] + a[3] * a[4] + a[6] * a[7];
-    double c02 = a[0] * a[2] + a[3] * a[5] + a[6] * a[8];
-    double c12 = a[1] * a[2] + a[4] * a[5] + a[7] * a[8];
-    double l0, l1, l2;
-    const double p1 = c01 * c01 + c02 * c02 + c12 * c12;
-    if (p1 == 0.0) {
-        l0 = fmax(c00, fmax(c11, c22));
-        l2 = fmin(c00, fmin(c11, c22));
-        l1 = c00 + c11 + c22 - l0 - l2;
-    } else {
-        const double q = (c00 + c11 + c22) / 3.0;
-        const double d0 = c00 - q, d1 = c11 - q, d2 = c22 - q;
-        const double p = sqrt((d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * p1) / 6.0);
-        const double ip = 1.0 / p;
-        const double b00 = d0 * ip, b11 = d1 * ip, b22 = d2 * ip, b01 = c01 * ip, b02 = c02 * ip, b12 = c12 * ip;
-        double hr = 0.5 * (b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) + b02 * (b01 * b12 - b11 * b02));
-        hr = fmin(1.0, fmax(-1.0, hr));
-        const double phi = acos(hr) / 3.0;
-        l0 = q + 2.0 * p * cos(phi);
-        l2 = q + 2.0 * p * cos(phi + 2.0943951023931954923);  // + 2 pi / 3
-        l1 = 3.0 * q - l0 - l2;
-    }
-    sig[0] = sqrt(fmax(l0, 0.0));
-    sig[1] = sqrt(fmax(l1, 0.0));
-    sig[2] = copysign(sqrt(fmax(l2, 0.0)), det3d(a));
-}
-
 // linear_elasticity.slang:135-148 in FP64: tau = 2 mu (F - R) F^T + lambda (J - 1) J I
 // with R the polar rotation (U V^T of the SVD), J = det F.
 __device__ inline void fc_stress_f64(float lambda, float mu, const float *f32, float *st) {
@@ -379,11 +344,19 @@ __device__ inline void fc_stress_f64(float lambda, float mu, const float *f32, f
         }
 }
 
-// drucker_prager.slang:144-200 in FP64 with the full SVD (cone projection).
-__device__ inline void dp_project_f64_svd(float *state, const float4 hp, double lambda, double mu, double alpha,
-                                          const double *f, float *f32) {
-    double u[9], s[3], v[9], nf[9];
+// drucker_prager.slang:144-200 in FP64 (Klar et al. 2016 return mapping in
+// log-strain space on the singular values).
+__device__ inline void dp_project_f64(float *state, const float4 hp, float lambda_f, float mu_f, float *f32) {
+    if (lambda_f == 0.0f) return;
+    const double lambda = lambda_f, mu = mu_f;
+    double f[9], u[9], s[3], v[9], nf[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) f[k] = f32[k];
     svd3<double>(f, u, s, v);
+    double q = state[1];
+    double angle = (double)hp.x + ((double)hp.y * q - (double)hp.w) * exp(-(double)hp.z * q);
+    double sa = sin(angle);
+    double alpha = sqrt(2.0 / 3.0) * (2.0 * sa) / (3.0 - sa);
     const double d = 3.0;
     double strain[3], dev[3], ns[3], hard;
 #pragma unroll
@@ -410,48 +383,6 @@ __device__ inline void dp_project_f64_svd(float *state, const float4 hp, double 
     svd_recompose<double>(u, ns, v, nf);
 #pragma unroll
     for (int k = 0; k < 9; k++) f32[k] = (float)nf[k];
-}
-
-// drucker_prager.slang:144-200 in FP64. The singular values decide the case; only
-// the cone projection needs U and V (full SVD). Inside the cone nothing changes;
-// under tension F becomes its polar rotation U V^T.
-__device__ inline void dp_project_f64(float *state, const float4 hp, float lambda_f, float mu_f, float *f32) {
-    if (lambda_f == 0.0f) return;
-    const double lambda = lambda_f, mu = mu_f;
-    double f[9], s[3];
-#pragma unroll
-    for (int k = 0; k < 9; k++) f[k] = f32[k];
-    double q = state[1];
-    double angle = (double)hp.x + ((double)hp.y * q - (double)hp.w) * exp(-(double)hp.z * q);
-    double sa = sin(angle);
-    double alpha = sqrt(2.0 / 3.0) * (2.0 * sa) / (3.0 - sa);
-    singular_values_f64(f, s);
-    const double d = 3.0;
-    double strain[3], dev[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) strain[i] = log(s[i]) + (double)state[2] / d;
-    double tr = strain[0] + strain[1] + strain[2];
-#pragma unroll
-    for (int i = 0; i < 3; i++) dev[i] = strain[i] - tr / d;
-    if (tr > 0.0 || (dev[0] == 0.0 && dev[1] == 0.0 && dev[2] == 0.0)) {
-        double r[9];
-        if (!polar_f64(f, r)) {
-            dp_project_f64_svd(state, hp, lambda, mu, alpha, f, f32);
-            return;
-        }
-        const double hard = sqrt(strain[0] * strain[0] + strain[1] * strain[1] + strain[2] * strain[2]);
-        const double prev_det = s[0] * s[1] * s[2];
-        state[0] = (float)((double)state[0] * prev_det / 1.0);
-        state[2] = (float)((double)state[2] + log(prev_det) - log(1.0));
-        state[1] = (float)((double)state[1] + hard);
-#pragma unroll
-        for (int k = 0; k < 9; k++) f32[k] = (float)r[k];
-        return;
-    }
-    double dn = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
-    double gamma = dn + (d * lambda + 2.0 * mu) / (2.0 * mu) * tr * alpha;
-    if (gamma <= 0.0) return;
-    dp_project_f64_svd(state, hp, lambda, mu, alpha, f, f32);
 }
 
 // ---------------------------------------------------------------------------
@@ -669,7 +600,7 @@ __device__ inline void g2p_particle(const float4 *__restrict__ sh, float h, floa
 // products and sums round like the shader source's separate operations. MODELS is the set of
 // material tags the kernel instance handles (bit t = tag t), so scenes without
 // SVD-based models do not carry that code's register footprint.
-template <uint32_t MODELS>
+template <uint32_t MODELS, bool F64>
 __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 &v, float &mass, uint32_t &enabled,
                                        float *aff, float *F, float4 prop, float4 model, const PStatic &st,
                                        uint32_t oid, float *dps) {
@@ -702,15 +633,15 @@ __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 
     if ((MODELS & 2u) && tag == 1u) {
         nh_stress(model.y, model.z, F, stress);
     } else if ((MODELS & 1u) && tag == 0u) {
-        if (P.svd_f64) fc_stress_f64(model.y, model.z, F, stress);
+        if (F64) fc_stress_f64(model.y, model.z, F, stress);
         else fc_stress(model.y, model.z, F, stress);
     } else if (((MODELS & 4u) && tag == 2u) || ((MODELS & 8u) && tag == 3u)) {
         float4 hp = P.uniform ? P.u_dp0 : st.dp0[oid];
         float4 pl = P.uniform ? P.u_dp1 : st.dp1[oid];
-        if (P.svd_f64) dp_project_f64(dps, hp, pl.x, pl.y, F);
+        if (F64) dp_project_f64(dps, hp, pl.x, pl.y, F);
         else dp_project(dps, hp, pl.x, pl.y, F);
         if (tag == 3u) nh_stress(model.y, model.z, F, stress);
-        else if (P.svd_f64) fc_stress_f64(model.y, model.z, F, stress);
+        else if (F64) fc_stress_f64(model.y, model.z, F, stress);
         else fc_stress(model.y, model.z, F, stress);
     } else {
 #pragma unroll
@@ -1081,8 +1012,10 @@ __device__ inline void load_halo(float4 *__restrict__ sh, const float4 *__restri
     }
 }
 
-template <uint32_t MODELS, int NT>
-__global__ __launch_bounds__(NT) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
+// F64: the SVD-based steps in FP64. Their register footprint would otherwise cap
+// the kernel at one wave per SIMD; the bound asks for two (a few spills).
+template <uint32_t MODELS, int NT, bool F64>
+__global__ __launch_bounds__(NT, F64 ? 2 : 1) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
                                                     float *__restrict__ fdt, const uint32_t *__restrict__ perm,
                                                     const uint32_t *__restrict__ blk_start,
                                                     const uint32_t *__restrict__ blk_cnt,
@@ -1126,7 +1059,7 @@ __global__ __launch_bounds__(NT) void k_g2p_update(StepParams P, PBuf in, PBuf o
 #pragma unroll
                 for (int k = 0; k < 3; k++) dps[k] = in.dps[(size_t)k * pcap + p];
             }
-            update_particle<MODELS>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
+            update_particle<MODELS, F64>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
             out.pos[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
             out.vm[slot] = make_float4(v.x, v.y, v.z, mass);
 #pragma unroll
@@ -1191,8 +1124,8 @@ __global__ __launch_bounds__(256) void k_g2p(StepParams P, PBuf buf, uint32_t pc
 }
 
 // Particle update only, in place (hook path), over every particle.
-template <uint32_t MODELS>
-__global__ __launch_bounds__(256) void k_particle_update(StepParams P, PBuf buf, PStatic st, uint32_t pcap,
+template <uint32_t MODELS, bool F64>
+__global__ __launch_bounds__(256, F64 ? 2 : 1) void k_particle_update(StepParams P, PBuf buf, PStatic st, uint32_t pcap,
                                                          float *__restrict__ fdt) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
@@ -1217,7 +1150,7 @@ __global__ __launch_bounds__(256) void k_particle_update(StepParams P, PBuf buf,
     }
     const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
     const float4 model = P.uniform ? P.u_model : st.model[oid];
-    update_particle<MODELS>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
+    update_particle<MODELS, F64>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
     buf.pos[p] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
     buf.vm[p] = make_float4(v.x, v.y, v.z, mass);
 #pragma unroll
@@ -1347,8 +1280,9 @@ hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, con
     st.dp1 = s.dp1;
     const bool nh_only = P.tag_mask == 2u;  // neo-Hookean only: no SVD code in the kernel
     const bool narrow = P.g2p_threads == 128u;
-    auto kern = nh_only ? (narrow ? k_g2p_update<2u, 128> : k_g2p_update<2u, 256>)
-                        : (narrow ? k_g2p_update<15u, 128> : k_g2p_update<15u, 256>);
+    auto kern = nh_only ? (narrow ? k_g2p_update<2u, 128, false> : k_g2p_update<2u, 256, false>)
+                        : (P.svd_f64 ? (narrow ? k_g2p_update<15u, 128, true> : k_g2p_update<15u, 256, true>)
+                                     : (narrow ? k_g2p_update<15u, 128, false> : k_g2p_update<15u, 256, false>));
     launch_kernel(ev, kern, dim3(block_grid(P, 1)), dim3(narrow ? 128 : 256), 0, stream, P, to_pbuf(in),
                   to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes,
                   t.counters, nt, b);
@@ -1370,12 +1304,9 @@ hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b,
     st.model = s.model;
     st.dp0 = s.dp0;
     st.dp1 = s.dp1;
-    if (P.tag_mask == 2u)
-        launch_kernel(ev, k_particle_update<2u>, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st,
-                      pcap, fdt);
-    else
-        launch_kernel(ev, k_particle_update<15u>, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st,
-                      pcap, fdt);
+    auto kern = P.tag_mask == 2u ? k_particle_update<2u, false>
+                                 : (P.svd_f64 ? k_particle_update<15u, true> : k_particle_update<15u, false>);
+    launch_kernel(ev, kern, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st, pcap, fdt);
     return hipGetLastError();
 }
 

@@ -127,7 +127,7 @@ def _ball_points(n, radius, center, seed):
     return (np.asarray(center) + d * r[:, None]).astype(np.float32)
 
 
-def sand_spheres(n_sand=2097152, n_per_sphere=32768, seed=SEED_BASE + 2, grid_capacity=1 << 17):
+def sand_spheres(n_sand=2097152, n_per_sphere=32768, seed=SEED_BASE + 2, grid_capacity=1 << 17, n_spheres=64):
     dx = 1.0 / 512
     u = uniform(seed, n_sand * 6).reshape(-1, 6)
     lo = np.array([0.375, 0.0, 0.375])
@@ -137,7 +137,7 @@ def sand_spheres(n_sand=2097152, n_per_sphere=32768, seed=SEED_BASE + 2, grid_ca
     sand = M.particle_arrays(sand_pos, np.float32(dx / 4), 1600.0, M.sand(3.5e5, 0.3, 30.0), sand_vel)
     parts = [sand]
     nh = M.elastic_neo_hookean(2.0e5, 0.4)
-    for s in range(64):
+    for s in range(n_spheres):
         i, j, k = s % 4, (s // 4) % 4, s // 16
         c = (0.5 + (i - 1.5) * 0.07, 0.09 + j * 0.07, 0.5 + (k - 1.5) * 0.07)
         sp = _ball_points(n_per_sphere, 0.03, c, seed * 131 + s + 1)
