@@ -261,10 +261,13 @@ __device__ inline void dp_project(float *state, const float4 hp, float lambda, f
     svd_recompose(u, ns, v, f);
 }
 
-// FP64 variants of the SVD-dependent steps: the SVD, the fixed-corotated F - R
-// assembly and the Drucker-Prager return mapping run in double and are rounded
-// once. In FP32, F - R and log(sigma) near sigma = 1 lose ~5 digits to
-// cancellation, which is the size of the parity tolerance.
+// Refined variants of the SVD-dependent steps. In FP32, F - R and log(sigma)
+// near sigma = 1 lose ~5 digits to cancellation, which is the size of the parity
+// tolerance. Here the quantities that cancel are computed to FP64 accuracy: the
+// polar rotation R by FP64 Newton iterations from F; the singular values as
+// |F v_i| / |v_i| in FP64 from the FP32 SVD's right singular vectors (the error
+// is second order in theirs); log(sigma) and exp of the small log-strain
+// corrections through log1p / expm1 of FP64 differences from 1.
 // Orthogonal polar factor r of a (a = r s, s symmetric positive definite) by
 // Newton's iteration x <- (g x + x^-T / g) / 2 with determinant scaling
 // g = det(x)^(-1/3) while det(x) is far from 1 (N. J. Higham, "Computing the
@@ -290,7 +293,8 @@ __device__ inline bool polar_f64(const double *a, double *r) {
         c[8] = x[0] * x[4] - x[1] * x[3];
         const double det = x[0] * c[0] + x[1] * c[1] + x[2] * c[2];
         if (!(det > 0.0)) return false;
-        const double g = fabs(det - 1.0) > 1.0e-3 ? 1.0 / cbrt(det) : 1.0;
+        // Any positive scale keeps the iteration convergent; FP32 is enough for it.
+        const double g = fabs(det - 1.0) > 1.0e-3 ? (double)(1.0f / cbrtf((float)det)) : 1.0;
         const double h = 0.5 / (g * det);
         double diff = 0.0, norm = 0.0;
 #pragma unroll
@@ -322,17 +326,14 @@ __device__ inline void fc_stress_f64(float lambda, float mu, const float *f32, f
 #pragma unroll
     for (int k = 0; k < 9; k++) f[k] = f32[k];
     double r[9];
-    if (polar_f64(f, r)) {
-#pragma unroll
-        for (int k = 0; k < 9; k++) rc[k] = f[k] - r[k];
-        j = det3d(f);
-    } else {
-        double u[9], s[3], v[9];
-        svd3<double>(f, u, s, v);
-        j = s[0] * s[1] * s[2];
-        double sm1[3] = {s[0] - 1.0, s[1] - 1.0, s[2] - 1.0};
-        svd_recompose<double>(u, sm1, v, rc);
+    if (!polar_f64(f, r)) {
+        // det F <= 0 (the particle is disabled by the update) or no convergence.
+        fc_stress(lambda, mu, f32, st);
+        return;
     }
+#pragma unroll
+    for (int k = 0; k < 9; k++) rc[k] = f[k] - r[k];
+    j = det3d(f);
     const double mu2 = 2.0 * (double)mu;
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -344,45 +345,73 @@ __device__ inline void fc_stress_f64(float lambda, float mu, const float *f32, f
         }
 }
 
-// drucker_prager.slang:144-200 in FP64 (Klar et al. 2016 return mapping in
-// log-strain space on the singular values).
+// log(x) for x near 1 from the FP64 difference x - 1.
+__device__ inline double log_near1(double x) { return (double)log1pf((float)(x - 1.0)); }
+
+// drucker_prager.slang:144-200 (Klar et al. 2016 return mapping in log-strain
+// space on the singular values), refined as described above.
 __device__ inline void dp_project_f64(float *state, const float4 hp, float lambda_f, float mu_f, float *f32) {
     if (lambda_f == 0.0f) return;
     const double lambda = lambda_f, mu = mu_f;
-    double f[9], u[9], s[3], v[9], nf[9];
+    float u[9], s32[3], v[9];
+    svd3<float>(f32, u, s32, v);
+    double s[3];
 #pragma unroll
-    for (int k = 0; k < 9; k++) f[k] = f32[k];
-    svd3<double>(f, u, s, v);
-    double q = state[1];
-    double angle = (double)hp.x + ((double)hp.y * q - (double)hp.w) * exp(-(double)hp.z * q);
-    double sa = sin(angle);
-    double alpha = sqrt(2.0 / 3.0) * (2.0 * sa) / (3.0 - sa);
+    for (int i = 0; i < 3; i++) {
+        const double v0 = v[0 * 3 + i], v1 = v[1 * 3 + i], v2 = v[2 * 3 + i];
+        const double w0 = (double)f32[0] * v0 + (double)f32[1] * v1 + (double)f32[2] * v2;
+        const double w1 = (double)f32[3] * v0 + (double)f32[4] * v1 + (double)f32[5] * v2;
+        const double w2 = (double)f32[6] * v0 + (double)f32[7] * v1 + (double)f32[8] * v2;
+        s[i] = sqrt((w0 * w0 + w1 * w1 + w2 * w2) / (v0 * v0 + v1 * v1 + v2 * v2));
+    }
+    if (s32[2] < 0.0f) s[2] = -s[2];  // svd3 gives sig[2] the sign of det F
+    const float q = state[1];
+    const float angle = hp.x + (hp.y * q - hp.w) * expf(-hp.z * q);
+    const float sa = sinf(angle);
+    const double alpha = (double)(sqrtf(2.0f / 3.0f) * (2.0f * sa) / (3.0f - sa));
     const double d = 3.0;
     double strain[3], dev[3], ns[3], hard;
 #pragma unroll
-    for (int i = 0; i < 3; i++) strain[i] = log(s[i]) + (double)state[2] / d;
+    for (int i = 0; i < 3; i++) strain[i] = log_near1(s[i]) + (double)state[2] / d;
     double tr = strain[0] + strain[1] + strain[2];
 #pragma unroll
     for (int i = 0; i < 3; i++) dev[i] = strain[i] - tr / d;
+    const double prev_det = s[0] * s[1] * s[2];
     if (tr > 0.0 || (dev[0] == 0.0 && dev[1] == 0.0 && dev[2] == 0.0)) {
-        ns[0] = ns[1] = ns[2] = 1.0;
+        // Tension: sigma -> 1, so F becomes its polar rotation U V^T.
         hard = sqrt(strain[0] * strain[0] + strain[1] * strain[1] + strain[2] * strain[2]);
-    } else {
-        double dn = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
-        double gamma = dn + (d * lambda + 2.0 * mu) / (2.0 * mu) * tr * alpha;
-        if (gamma <= 0.0) return;
+        double fd[9], r[9];
 #pragma unroll
-        for (int i = 0; i < 3; i++) ns[i] = exp(strain[i] - dev[i] * (gamma / dn));
-        hard = gamma;
+        for (int k = 0; k < 9; k++) fd[k] = f32[k];
+        if (polar_f64(fd, r)) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) f32[k] = (float)r[k];
+        } else {
+            const float one[3] = {1.0f, 1.0f, 1.0f};
+            svd_recompose(u, one, v, f32);
+        }
+        state[0] = (float)((double)state[0] * prev_det);
+        state[2] = (float)((double)state[2] + log_near1(prev_det));
+        state[1] = (float)((double)state[1] + hard);
+        return;
     }
-    double prev_det = s[0] * s[1] * s[2];
-    double new_det = ns[0] * ns[1] * ns[2];
-    state[0] = (float)((double)state[0] * prev_det / new_det);
-    state[2] = (float)((double)state[2] + log(prev_det) - log(new_det));
-    state[1] = (float)((double)state[1] + hard);
-    svd_recompose<double>(u, ns, v, nf);
+    double dn = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+    double gamma = dn + (d * lambda + 2.0 * mu) / (2.0 * mu) * tr * alpha;
+    if (gamma <= 0.0) return;
 #pragma unroll
-    for (int k = 0; k < 9; k++) f32[k] = (float)nf[k];
+    for (int i = 0; i < 3; i++) ns[i] = 1.0 + (double)expm1f((float)(strain[i] - dev[i] * (gamma / dn)));
+    hard = gamma;
+    const double new_det = ns[0] * ns[1] * ns[2];
+    state[0] = (float)((double)state[0] * prev_det / new_det);
+    state[2] = (float)((double)state[2] + log_near1(prev_det) - log_near1(new_det));
+    state[1] = (float)((double)state[1] + hard);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            f32[i * 3 + j] = (float)((double)u[i * 3 + 0] * ns[0] * (double)v[j * 3 + 0] +
+                                     (double)u[i * 3 + 1] * ns[1] * (double)v[j * 3 + 1] +
+                                     (double)u[i * 3 + 2] * ns[2] * (double)v[j * 3 + 2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1012,10 +1041,10 @@ __device__ inline void load_halo(float4 *__restrict__ sh, const float4 *__restri
     }
 }
 
-// F64: the SVD-based steps in FP64. Their register footprint would otherwise cap
-// the kernel at one wave per SIMD; the bound asks for two (a few spills).
+// F64: the SVD-based steps refined to FP64 accuracy. The bound keeps that
+// variant at three waves per SIMD (168 VGPRs).
 template <uint32_t MODELS, int NT, bool F64>
-__global__ __launch_bounds__(NT, F64 ? 2 : 1) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
+__global__ __launch_bounds__(NT, F64 ? 3 : 1) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
                                                     float *__restrict__ fdt, const uint32_t *__restrict__ perm,
                                                     const uint32_t *__restrict__ blk_start,
                                                     const uint32_t *__restrict__ blk_cnt,
@@ -1125,7 +1154,7 @@ __global__ __launch_bounds__(256) void k_g2p(StepParams P, PBuf buf, uint32_t pc
 
 // Particle update only, in place (hook path), over every particle.
 template <uint32_t MODELS, bool F64>
-__global__ __launch_bounds__(256, F64 ? 2 : 1) void k_particle_update(StepParams P, PBuf buf, PStatic st, uint32_t pcap,
+__global__ __launch_bounds__(256, F64 ? 3 : 1) void k_particle_update(StepParams P, PBuf buf, PStatic st, uint32_t pcap,
                                                          float *__restrict__ fdt) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
