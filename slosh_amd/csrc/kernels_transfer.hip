@@ -28,7 +28,7 @@ constexpr int ASSOC[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1},
                              {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
 
 // NBH_SHIFTS order of kernel.slang:120-149 (summation order of G2P).
-__constant__ uint8_t NBH[27][3] = {
+constexpr uint8_t NBH_C[27][3] = {
     {2, 2, 2}, {2, 0, 2}, {2, 1, 2}, {0, 2, 2}, {0, 0, 2}, {0, 1, 2}, {1, 2, 2}, {1, 0, 2}, {1, 1, 2},
     {2, 2, 0}, {2, 0, 0}, {2, 1, 0}, {0, 2, 0}, {0, 0, 0}, {0, 1, 0}, {1, 2, 0}, {1, 0, 0}, {1, 1, 0},
     {2, 2, 1}, {2, 0, 1}, {2, 1, 1}, {0, 2, 1}, {0, 0, 1}, {0, 1, 1}, {1, 2, 1}, {1, 0, 1}, {1, 1, 1}};
@@ -575,17 +575,17 @@ struct PStatic {
     const float4 *dp1;   // by original id: (lambda, mu, 0, 0) of the plastic part
 };
 
-// G2P of one particle from the 6x6x6 LDS halo (g2p.slang:150-265).
+// G2P of one particle from the 6x6x6 LDS halo (g2p.slang:150-265):
+// v = sum w v_i, C = (4/h^2) sum w v_i dpt_i^T, vel_grad_det = (4/h^2) sum w v_i . dpt_i.
+// With dpt_i = ref + s_i h (s_i in {0,1,2}^3 the node's stencil shift), the sums
+// are formed from V = sum w v_i and S_a = sum w s_i,a v_i (a = x, y, z):
+// sum w v_i dpt_i,a = V ref_a + h S_a.
 __device__ inline void g2p_particle(const float4 *__restrict__ sh, float h, float3 p, bool enabled, float3 *vel,
                                     float *vg, float *vgd_out) {
-    float4 mvm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float g[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) g[k] = 0.0f;
-    float vgd = 0.0f;
+    float V[3] = {0.0f, 0.0f, 0.0f}, S[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    float ref[3] = {0.0f, 0.0f, 0.0f};
     if (enabled) {
-        float inv_d = 4.0f / (h * h);
-        float ref[3], w[3][3];
+        float w[3][3];
         float pp[3] = {p.x, p.y, p.z};
         uint32_t lc[3];
 #pragma unroll
@@ -596,30 +596,35 @@ __device__ inline void g2p_particle(const float4 *__restrict__ sh, float h, floa
             float ab = floorf(ac / 4.0f) * 4.0f;
             lc[i] = (uint32_t)(ac - ab);
         }
-        uint32_t packed = lc[0] + lc[1] * 6u + lc[2] * 36u;
+        const uint32_t packed = lc[0] + lc[1] * 6u + lc[2] * 36u;
 #pragma unroll
         for (int i = 0; i < 27; i++) {
-            const uint32_t sx = NBH[i][0], sy = NBH[i][1], sz = NBH[i][2];
-            float4 cd = sh[packed + sx + sy * 6u + sz * 36u];
-            float dpt[3] = {ref[0] + (float)sx * h, ref[1] + (float)sy * h, ref[2] + (float)sz * h};
-            float weight = w[0][sx] * w[1][sy] * w[2][sz];
-            mvm.x += cd.x * weight;
-            mvm.y += cd.y * weight;
-            mvm.z += cd.z * weight;
-            mvm.w += cd.w * weight;
-            float wi = weight * inv_d;
-            float cv[3] = {cd.x, cd.y, cd.z};
+            const uint32_t sx = NBH_C[i][0], sy = NBH_C[i][1], sz = NBH_C[i][2];
+            const float4 cd = sh[packed + sx + sy * 6u + sz * 36u];
+            const float weight = w[0][sx] * w[1][sy] * w[2][sz];
+            const float wv[3] = {cd.x * weight, cd.y * weight, cd.z * weight};
 #pragma unroll
-            for (int r = 0; r < 3; r++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) g[r * 3 + c] += wi * (cv[c] * dpt[r]);
-            vgd += weight * inv_d * (cd.x * dpt[0] + cd.y * dpt[1] + cd.z * dpt[2]);
+            for (int c = 0; c < 3; c++) {
+                V[c] += wv[c];
+                if (sx == 1) S[0][c] += wv[c];
+                if (sx == 2) S[0][c] += 2.0f * wv[c];
+                if (sy == 1) S[1][c] += wv[c];
+                if (sy == 2) S[1][c] += 2.0f * wv[c];
+                if (sz == 1) S[2][c] += wv[c];
+                if (sz == 2) S[2][c] += 2.0f * wv[c];
+            }
         }
     }
-    *vel = make_float3(mvm.x, mvm.y, mvm.z);
+    *vel = make_float3(V[0], V[1], V[2]);
+    const float inv_d = 4.0f / (h * h);
+    float vgd = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 9; k++) vg[k] = g[k];
-    *vgd_out = vgd;
+    for (int r = 0; r < 3; r++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) vg[r * 3 + c] = inv_d * (V[c] * ref[r] + h * S[r][c]);
+        vgd += V[r] * ref[r] + h * S[r][r];
+    }
+    *vgd_out = inv_d * vgd;
 }
 
 // particle_update.slang:32-160 for one particle. `aff` holds the velocity

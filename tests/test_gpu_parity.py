@@ -19,9 +19,11 @@ pytestmark = pytest.mark.gpu
 TOL = {1: 1.0e-5, 100: 1.0e-3}
 
 
-def run_both(scene, steps, gpu_pipeline, svd_mode=1):
+def run_both(scene, steps, gpu_pipeline, svd_mode=1, tuning=None):
     pl, st = gpu_pipeline
     data = P.MpmData.from_scene(pl, scene)
+    if tuning:
+        data.set_tuning(**tuning)
     orc = O.Oracle(scene, svd_mode=svd_mode)
     for _ in range(steps):
         pl.launch_step(data, st)
@@ -102,6 +104,21 @@ def test_parity_mixed(gpu_pipeline, steps):
     assert np.allclose(gp["translation_scale"], op["translation_scale"], rtol=1e-5, atol=1e-6)
     assert np.allclose(gp["rotation"], op["rotation"], rtol=1e-5, atol=1e-6)
     assert np.allclose(gv["linear"], ov["linear"], rtol=1e-5, atol=1e-6)
+
+
+# Every launch shape computes the same substep (P2G staging rounds of 256 / 512 /
+# 1024 particles, G2P workgroups of 128 / 256, persistent widths).
+SHAPES = [{"p2g_chunk": 512, "g2p_threads": 256, "block_wgs": 4096},
+          {"p2g_chunk": 256, "g2p_threads": 128, "block_wgs": 2048},
+          {"p2g_chunk": 1024, "g2p_threads": 256, "block_wgs": 16384}]
+
+
+@pytest.mark.parametrize("shape", range(len(SHAPES)))
+@pytest.mark.parametrize("steps", [1, 100])
+def test_parity_mixed_launch_shapes(gpu_pipeline, steps, shape):
+    sc = scenes.mixed_test_scene()
+    data, orc = run_both(sc, steps, gpu_pipeline, tuning=SHAPES[shape])
+    check_state(sc, data, orc, gpu_pipeline[1], steps)
 
 
 @pytest.mark.parametrize("steps", [1, 100])
