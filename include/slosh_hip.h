@@ -233,6 +233,16 @@ int slosh_set_svd_fp64(slosh_data *data, int enable);
 
 /* Rewrites SimulationParams (cf. src_testbed/step.rs:136-147). */
 int slosh_set_sim_params(slosh_data *data, const slosh_sim_params *params);
+/* WgTimestepBounds::compute_bounds (src/solver/timestep_bound.rs:56-79): the
+ * largest stable timestep estimate over the enabled particles, in seconds
+ * (computed_dt of the u32 fixed point in 1e-12 s units, also returned in raw if
+ * non-null). Synchronous on `stream`. */
+int slosh_timestep_bound(slosh_data *data, slosh_stream stream, float *out_dt, uint32_t *out_raw);
+/* The adaptive substep count of src_testbed/step.rs:100-147: with
+ * min_substeps < max_substeps, ceil(base_dt / bound) clamped to the range;
+ * otherwise max_substeps. Sets the simulation dt to base_dt / count. */
+int slosh_adaptive_substeps(slosh_data *data, slosh_stream stream, float base_dt, uint32_t min_substeps,
+                            uint32_t max_substeps, uint32_t *out_substeps);
 
 /* Synchronizes `stream`, returns SLOSH_E_GRID_OVERFLOW if any substep since the
  * last call overflowed the sparse-grid hash map, and clears the flag. */

@@ -47,6 +47,10 @@ def lib():
         l.so_pack_key.restype = C.c_uint32
         l.so_hash.argtypes = [C.c_uint32]
         l.so_hash.restype = C.c_uint32
+        l.so_timestep_bound.argtypes = [vp]
+        l.so_timestep_bound.restype = C.c_uint32
+        l.so_particle_timestep_bound.argtypes = [vp, vp, vp, vp, C.c_float]
+        l.so_particle_timestep_bound.restype = C.c_uint32
         _lib = l
     return _lib
 
@@ -113,6 +117,13 @@ class Oracle:
             lib().so_read_bodies(self._s, _p(poses), _p(vels))
         return poses, vels
 
+    def timestep_bound_raw(self):
+        return int(lib().so_timestep_bound(self._s))
+
+    def timestep_bound(self):
+        """Seconds, as GpuTimestepBounds::computed_dt (timestep_bound.rs:26-29)."""
+        return float(np.float32(self.timestep_bound_raw()) / np.float32(1.0e12))
+
     def blocks(self):
         nb = lib().so_num_blocks(self._s)
         out = np.zeros(nb, L.BLOCK_INFO)
@@ -143,6 +154,15 @@ def model_update(model, def_grad, svd_mode=1):
     st = np.zeros(9, np.float32)
     lib().so_model_update(_p(m), _p(f), _p(st), svd_mode)
     return m, f.reshape(3, 3), st.reshape(3, 3)
+
+
+def particle_timestep_bound(model, kin, def_grad, props, cell_width):
+    """One particle's bound (u32 fixed point, 1e-12 s units)."""
+    m = np.array(model, dtype=np.uint32).reshape(13).copy()
+    k = np.array([kin], dtype=L.KINEMATICS)
+    f = np.ascontiguousarray(def_grad, dtype=np.float32).reshape(1, 3, 4).copy()
+    pr = np.array([props], dtype=L.PROPERTIES)
+    return int(lib().so_particle_timestep_bound(_p(m), _p(k), _p(f), _p(pr), float(cell_width)))
 
 
 def prefix_sum(v):

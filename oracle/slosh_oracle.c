@@ -1266,6 +1266,56 @@ void so_read_bodies(const so_state *s, so_sim3 *poses, so_velocity *vels) {
         if (vels) vels[i] = s->bodies[i].vel;
     }
 }
+/* ------------------------------------------------------------------------- */
+/* Timestep bound, WgTimestepBounds (src/solver/timestep_bound.rs:56-102) and     */
+/* shaders/slosh/solver/timestep_bound.slang:18-92 with the models' sound-speed   */
+/* bound (models/utils.slang:100-136; default.slang:94-127).                      */
+
+/* GpuTimestepBounds::secs_to_int (timestep_bound.slang:21-24), saturating. */
+static uint32_t secs_to_int(float secs) {
+    float x = floorf(secs * 1.0e12f);
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967295.0f) return 0xffffffffu;
+    return (uint32_t)x;
+}
+
+uint32_t so_particle_timestep_bound(const so_model *model, const so_kinematics *kin, const so_mat3 *def_grad,
+                                    const so_props *props, float cell_width) {
+    float density0 = kin->mass / props->init_volume;
+    m3 f = m3_load(def_grad);
+    float det = m3_det(f);
+    uint32_t tag = model->w[0];
+    float dt;
+    if (tag < 4u) {
+        /* elastic part: (lambda, mu, cfl) at words 1..3 (elastic) or 10..12 (sand) */
+        const uint32_t *e = model->w + (tag < 2u ? 1 : 10);
+        float lambda, mu, cfl;
+        memcpy(&lambda, e + 0, 4);
+        memcpy(&mu, e + 1, 4);
+        memcpy(&cfl, e + 2, 4);
+        float bulk = lambda + 2.0f * mu / 3.0f; /* bulk_modulus_from_lame, utils.slang:26-28 */
+        float shear = mu;                       /* shear_modulus_from_lame, utils.slang:22-24 */
+        float curr_density = density0 / fmaxf(det, 1.0e-6f);
+        float sound_speed = sqrtf((bulk + shear * 4.0f / 3.0f) / curr_density);
+        float max_speed = fmaxf(len3(kin->velocity), sound_speed);
+        dt = cfl * cell_width / max_speed;
+    } else {
+        dt = 0.0f; /* default.slang:124-125 */
+    }
+    dt = fminf(dt, cell_width / len3(kin->velocity));
+    return secs_to_int(dt);
+}
+
+uint32_t so_timestep_bound(const so_state *s) {
+    uint32_t best = 0xffffffffu; /* reset_timestep_bound */
+    for (size_t id = 0; id < s->n; id++) {
+        if (s->kin[id].enabled == 0) continue;
+        uint32_t c = so_particle_timestep_bound(&s->models[id], &s->kin[id], &s->F[id], &s->props[id], s->h);
+        if (c < best) best = c;
+    }
+    return best;
+}
+
 uint32_t so_num_blocks(const so_state *s) { return s->nblocks; }
 void so_read_blocks(const so_state *s, so_block_info *out) {
     for (uint32_t i = 0; i < s->nblocks; i++) {

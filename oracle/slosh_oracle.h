@@ -99,6 +99,11 @@ void so_svd3(const double a[9], int f64, double u[9], double sig[3], double v[9]
 /* Applies the material update of default.slang:51-92 to def_grad (row-major 3x3,
  * shader convention) and returns the Kirchhoff stress; model is updated in place. */
 void so_model_update(so_model *model, float def_grad[9], float stress[9], int svd_mode);
+/* Timestep bound (WgTimestepBounds): min over enabled particles of the sound-speed
+ * and velocity bounds, as the reference's u32 fixed point in 1e-12 s units. */
+uint32_t so_timestep_bound(const so_state *s);
+uint32_t so_particle_timestep_bound(const so_model *model, const so_kinematics *kin, const so_mat3 *def_grad,
+                                    const so_props *props, float cell_width);
 void so_prefix_sum(uint32_t *v, size_t len);
 /* Grid update of one node at cell_pos (update_single_cell + collide + project_velocity),
  * mvm = (momentum.xyz, mass) in, (velocity.xyz, mass) out; out_normal_dist = (n, dist). */

@@ -1128,6 +1128,52 @@ __global__ __launch_bounds__(NT, F64 ? 3 : 1) void k_g2p_update(StepParams P, PB
     }
 }
 
+// WgTimestepBounds (timestep_bound.slang:18-92): per enabled particle, the
+// material's P-wave sound-speed bound cfl h / max(|v|, c) (models/utils.slang:
+// 119-135; elastic part of sand models, default.slang:94-127) and the velocity
+// bound h / |v|, as the u32 fixed point of GpuTimestepBounds::secs_to_int,
+// reduced with atomicMin into *out.
+__device__ inline uint32_t secs_to_int(float secs) {
+    const float x = floorf(secs * 1.0e12f);
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967295.0f) return 0xffffffffu;
+    return (uint32_t)x;
+}
+
+__global__ __launch_bounds__(256) void k_timestep_bound(StepParams P, PBuf buf, PStatic st, uint32_t pcap,
+                                                        uint32_t *__restrict__ out) {
+#pragma clang fp contract(off)
+    uint32_t best = 0xffffffffu;
+    const float h = P.h;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += gridDim.x * blockDim.x) {
+        const float4 ps = buf.pos[i];
+        if ((__float_as_uint(ps.w) & 1u) == 0u) continue;
+        const float4 vm = buf.vm[i];
+        const uint32_t oid = buf.ids[i];
+        const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
+        const float4 model = P.uniform ? P.u_model : st.model[oid];
+        float F[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) F[k] = buf.F[(size_t)k * pcap + i];
+        const float speed = sqrtf(vm.x * vm.x + vm.y * vm.y + vm.z * vm.z);
+        float dt = 0.0f;
+        if (__float_as_uint(model.x) < 4u) {
+            const float lambda = model.y, mu = model.z, cfl = model.w;
+            const float bulk = lambda + 2.0f * mu / 3.0f;
+            const float shear = mu;
+            const float density0 = vm.w / prop.x;
+            const float curr_density = density0 / fmaxf(det3(F), 1.0e-6f);
+            const float sound_speed = sqrtf((bulk + shear * 4.0f / 3.0f) / curr_density);
+            dt = cfl * h / fmaxf(speed, sound_speed);
+        }
+        dt = fminf(dt, h / speed);
+        best = min(best, secs_to_int(dt));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) best = min(best, (uint32_t)__shfl_xor((int)best, o));
+    if (lane_id() == 0 && best != 0xffffffffu) atomicMin(out, best);
+}
+
 // G2P only, in place (hook path): writes velocity, velocity gradient, vel_grad_det.
 __global__ __launch_bounds__(256) void k_g2p(StepParams P, PBuf buf, uint32_t pcap, const uint32_t *__restrict__ perm,
                                              const uint32_t *__restrict__ blk_start,
@@ -1341,6 +1387,19 @@ hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b,
     auto kern = P.tag_mask == 2u ? k_particle_update<2u, false>
                                  : (P.svd_f64 ? k_particle_update<15u, true> : k_particle_update<15u, false>);
     launch_kernel(ev, kern, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st, pcap, fdt);
+    return hipGetLastError();
+}
+
+hipError_t launch_timestep_bound(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
+                                 uint32_t *out, hipStream_t stream) {
+    if (P.n == 0) return hipSuccess;
+    PStatic st;
+    st.prop = s.prop;
+    st.model = s.model;
+    st.dp0 = s.dp0;
+    st.dp1 = s.dp1;
+    hipLaunchKernelGGL(k_timestep_bound, dim3(min_t(div_up_t(P.n, 256), 4096)), dim3(256), 0, stream, P, to_pbuf(b),
+                       st, pcap, out);
     return hipGetLastError();
 }
 

@@ -82,6 +82,8 @@ hipError_t launch_g2p(const StepParams &P, const ParticleBuffers &b, uint32_t pc
 hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
                                   float *fdt, const PassEvents *ev, hipStream_t stream);
 hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, const PassEvents *ev, hipStream_t stream);
+hipError_t launch_timestep_bound(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
+                                 uint32_t *out, hipStream_t stream);
 
 struct CompactArgs {
     float4 *pos[2], *vm[2];
@@ -198,6 +200,7 @@ struct slosh_data {
     uint32_t *hdr_key = nullptr, *nbr = nullptr, *cell_cnt = nullptr, *cell_start = nullptr;
     uint32_t *blk_start = nullptr, *blk_cnt = nullptr, *pslot = nullptr, *prank = nullptr, *perm = nullptr;
     uint32_t *blk_tile_sums = nullptr;
+    uint32_t *dt_bound = nullptr;  // GpuTimestepBounds word
     uint32_t *counters = nullptr;
     float4 *tiles = nullptr, *nodes = nullptr;
     float4 *coll[2] = {nullptr, nullptr};
@@ -800,6 +803,7 @@ static void free_data(slosh_data *d) {
     dfree(d->mats);
     dfree(d->impulses);
     dfree(d->dead);
+    dfree(d->dt_bound);
     if (d->capture_stream) (void)hipStreamDestroy(d->capture_stream);
     delete d;
 }
@@ -899,6 +903,7 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     TRY(dalloc(&d->counters, 16));
     HIP_TRY(hipMemset(d->counters, 0, 64));
     TRY(dalloc(&d->dead, SLOSH_MAX_SCENES + 1));
+    TRY(dalloc(&d->dt_bound, 1));
     TRY(dalloc(&d->tiles, cap * TILE_NODES));
     TRY(dalloc(&d->nodes, cap * NODES_PER_BLOCK));
     d->nbodies = (uint32_t)num_bodies;
@@ -1074,6 +1079,42 @@ int slosh_set_sim_params(slosh_data *d, const slosh_sim_params *params) {
     if (!d || !params) return fail(SLOSH_E_INVALID, "null argument");
     d->params = *params;
     invalidate_graphs(d);
+    return SLOSH_OK;
+}
+
+int slosh_timestep_bound(slosh_data *d, slosh_stream stream, float *out_dt, uint32_t *out_raw) {
+    if (!d || !out_dt) return fail(SLOSH_E_INVALID, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipMemsetD32Async(d->dt_bound, (int)0xffffffffu, 1, st));  // reset_timestep_bound
+    const StepParams P = make_params(d);
+    HIP_TRY(launch_timestep_bound(P, pbufs(d, d->cur_p), sbufs(d), (uint32_t)d->pcap, d->dt_bound, st));
+    uint32_t raw = 0xffffffffu;
+    HIP_TRY(hipMemcpyAsync(&raw, d->dt_bound, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *out_dt = (float)raw / 1.0e12f;  // GpuTimestepBounds::computed_dt
+    if (out_raw) *out_raw = raw;
+    return SLOSH_OK;
+}
+
+int slosh_adaptive_substeps(slosh_data *d, slosh_stream stream, float base_dt, uint32_t min_substeps,
+                            uint32_t max_substeps, uint32_t *out_substeps) {
+    if (!d || !out_substeps) return fail(SLOSH_E_INVALID, "null argument");
+    if (max_substeps == 0 || min_substeps > max_substeps) return fail(SLOSH_E_INVALID, "invalid substep range");
+    uint32_t n = max_substeps;
+    if (min_substeps < max_substeps) {
+        float bound = 0.0f;
+        TRY(slosh_timestep_bound(d, stream, &bound, nullptr));
+        const float q = ceilf(base_dt / bound);
+        // `as u32`: NaN -> 0, saturating at both ends.
+        const uint32_t est = !(q > 0.0f) ? 0u : (q >= 4294967295.0f ? 0xffffffffu : (uint32_t)q);
+        n = std::min(std::max(est, min_substeps), max_substeps);
+    }
+    const float dt = base_dt / (float)n;
+    if (dt != d->params.dt) {
+        d->params.dt = dt;
+        invalidate_graphs(d);
+    }
+    *out_substeps = n;
     return SLOSH_OK;
 }
 

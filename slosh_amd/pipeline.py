@@ -263,6 +263,21 @@ class MpmData:
         self._params = np.array([params], dtype=L.SIM_PARAMS)
         _lib.check(_lib.load().slosh_set_sim_params(self.handle, _p(self._params)))
 
+    def timestep_bound(self, stream=None):
+        """(seconds, raw u32 fixed point) -- WgTimestepBounds::compute_bounds."""
+        dt = C.c_float()
+        raw = C.c_uint32()
+        _lib.check(_lib.load().slosh_timestep_bound(self.handle, self._s(stream), C.byref(dt), C.byref(raw)))
+        return dt.value, raw.value
+
+    def adaptive_substeps(self, base_dt, min_substeps, max_substeps, stream=None):
+        """Substep count for one frame of length base_dt (src_testbed/step.rs:100-147);
+        sets the simulation dt to base_dt / count."""
+        n = C.c_uint32()
+        _lib.check(_lib.load().slosh_adaptive_substeps(self.handle, self._s(stream), float(base_dt),
+                                                       int(min_substeps), int(max_substeps), C.byref(n)))
+        return n.value
+
     def set_svd_fp64(self, enable):
         _lib.check(_lib.load().slosh_set_svd_fp64(self.handle, 1 if enable else 0))
 

@@ -227,3 +227,26 @@ def test_grid_overflow_is_reported(gpu_pipeline):
     pl.launch_step(data, st)
     with pytest.raises(Exception, match="OVERFLOW"):
         data.check_errors(st)
+
+
+@pytest.mark.parametrize("make", [scenes.mixed_test_scene, lambda: scenes.stir(scenes.elastic_cube(), 77)])
+def test_timestep_bound_matches_oracle(gpu_pipeline, make):
+    """WgTimestepBounds: bit-exact on identical inputs, within round-off after
+    substeps; the adaptive substep count of src_testbed/step.rs:100-147."""
+    pl, st = gpu_pipeline
+    sc = make()
+    data = P.MpmData.from_scene(pl, sc)
+    orc = O.Oracle(sc, svd_mode=1)
+    dt, raw = data.timestep_bound(st)
+    assert raw == orc.timestep_bound_raw() and dt == orc.timestep_bound()
+    for _ in range(10):
+        pl.launch_step(data, st)
+    orc.step(10)
+    dt, raw = data.timestep_bound(st)
+    assert abs(raw - orc.timestep_bound_raw()) <= 1e-5 * orc.timestep_bound_raw()
+    base = 1.0 / 60.0
+    n = data.adaptive_substeps(base, 1, 100000, st)
+    assert n == min(max(int(np.ceil(np.float32(base) / np.float32(dt))), 1), 100000)
+    assert data.adaptive_substeps(base, 20, 20, st) == 20
+    pl.launch_step(data, st)   # the new dt is used (graphs re-captured on the next replay)
+    data.check_errors(st)

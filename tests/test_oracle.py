@@ -374,3 +374,70 @@ def test_fixed_body_does_not_move():
     poses, vels = o.bodies()
     assert np.array_equal(poses["translation_scale"], sc.bodies["pose"]["translation_scale"])
     assert np.all(vels["linear"] == 0)
+
+
+# --- timestep bound (timestep_bound.slang:18-92) -------------------------------
+
+def _kin(mass, velocity=(0.0, 0.0, 0.0), enabled=1):
+    k = np.zeros(1, L.KINEMATICS)[0]
+    k["mass"] = mass
+    k["velocity"] = velocity
+    k["enabled"] = enabled
+    return k
+
+
+def _props(volume):
+    p = np.zeros(1, L.PROPERTIES)[0]
+    p["init_volume"] = volume
+    return p
+
+
+def _F(diag=(1.0, 1.0, 1.0)):
+    f = np.zeros((3, 4), np.float32)
+    f[0, 0], f[1, 1], f[2, 2] = diag
+    return f
+
+
+def _units(seconds):
+    return seconds * 1.0e12
+
+
+@pytest.mark.parametrize("make", [M.elastic, M.elastic_neo_hookean,
+                                  lambda e, nu: M.sand(e, nu, 30.0), lambda e, nu: M.sand(e, nu, 30.0, True)])
+def test_timestep_bound_is_the_p_wave_cfl_bound_at_rest(make):
+    # At rest with F = I the bound is cfl * h / c with c the P-wave speed
+    # sqrt(M / rho), M = E (1 - nu) / ((1 + nu)(1 - 2 nu)) (independent of the
+    # Lame split the model stores).
+    e, nu, rho, h, vol = 2.0e5, 0.3, 1000.0, 1.0 / 128, 1.0e-7
+    got = O.particle_timestep_bound(make(e, nu), _kin(rho * vol), _F(), _props(vol), h)
+    m = e * (1 - nu) / ((1 + nu) * (1 - 2 * nu))
+    want = _units(0.5 * h / np.sqrt(m / rho))
+    assert abs(got - want) <= 1e-5 * want
+
+
+def test_timestep_bound_compression_fast_particles_and_minimum():
+    e, nu, rho, h, vol = 1.0e5, 0.25, 1000.0, 1.0 / 64, 1.0e-6
+    model = M.elastic_neo_hookean(e, nu)
+    rest = O.particle_timestep_bound(model, _kin(rho * vol), _F(), _props(vol), h)
+    # Compressed by J = 0.81: the current density rho0 / J grows, so with fixed
+    # moduli the sound speed drops by sqrt(J) = 0.9 and the bound grows by 1 / 0.9.
+    comp = O.particle_timestep_bound(model, _kin(rho * vol), _F((0.9, 0.9, 1.0)), _props(vol), h)
+    assert abs(comp / rest - 1.0 / 0.9) < 1e-5
+    # Faster than sound: cfl * h / |v| (the velocity bound h / |v| is larger).
+    v = 100.0
+    fast = O.particle_timestep_bound(model, _kin(rho * vol, (0.0, v, 0.0)), _F(), _props(vol), h)
+    assert abs(fast - _units(0.5 * h / v)) <= 1e-5 * _units(0.5 * h / v)
+    # Unknown material tag: bound 0 (default.slang:124-125); huge bound saturates.
+    assert O.particle_timestep_bound(np.array([7] + [0] * 12, np.uint32), _kin(1.0), _F(), _props(1.0), h) == 0
+    assert O.particle_timestep_bound(model, _kin(1.0e12), _F(), _props(1.0), 10.0) == 0xFFFFFFFF
+    # Over a scene: the minimum over enabled particles.
+    sc = scenes.elastic_cube(n_side=4)
+    sc.particles["kin"]["velocity"][5] = (0.0, 50.0, 0.0)
+    sc.particles["kin"]["velocity"][9] = (0.0, 400.0, 0.0)
+    sc.particles["kin"]["enabled"][9] = 0
+    orc = O.Oracle(sc)
+    each = [O.particle_timestep_bound(sc.particles["models"][i], sc.particles["kin"][i], sc.particles["def_grad"][i],
+                                      sc.particles["props"][i], sc.cell_width)
+            for i in range(sc.n) if sc.particles["kin"]["enabled"][i]]
+    assert orc.timestep_bound_raw() == min(each)
+    assert abs(orc.timestep_bound() - 0.5 * float(sc.cell_width) / 50.0) < 1e-6 * 0.5 * float(sc.cell_width) / 50.0
