@@ -204,6 +204,17 @@ def run_batched(args):
     batched = batch.n * args.batch_steps / (ms * 1e-3)
     g = None
     data.close()
+    out = {
+        "workload": f"{len(scs)} scenes, {batch.n} particles (N log-uniform [2e3, 2e5]; NH / FC / DP)",
+        "value": batched,
+        "unit": "particle-substeps/s",
+        "steps": args.batch_steps,
+        "ms_per_step": ms / args.batch_steps,
+        "active_blocks": int(n_blk),
+        "tuning": tuning,
+    }
+    if not args.batch_loop:
+        return out
     # Plain per-scene loop over the same scenes and substeps.
     loop_ms = 0.0
     for sc in scs:
@@ -222,18 +233,9 @@ def run_batched(args):
         gs = None
         d.close()
     loop = batch.n * args.batch_steps / (loop_ms * 1e-3)
-    return {
-        "workload": f"{len(scs)} scenes, {batch.n} particles (N log-uniform [2e3, 2e5]; NH / FC / DP)",
-        "value": batched,
-        "unit": "particle-substeps/s",
-        "steps": args.batch_steps,
-        "ms_per_step": ms / args.batch_steps,
-        "active_blocks": int(n_blk),
-        "tuning": tuning,
-        "per_scene_loop": loop,
-        "per_scene_loop_ms_per_step": loop_ms / args.batch_steps,
-        "speedup_vs_per_scene_loop": batched / loop,
-    }
+    out.update({"per_scene_loop": loop, "per_scene_loop_ms_per_step": loop_ms / args.batch_steps,
+                "speedup_vs_per_scene_loop": batched / loop})
+    return out
 
 
 def run_served(args):
@@ -279,6 +281,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched leg (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--no-batch-loop", dest="batch_loop", action="store_false",
+                    help="skip the per-scene loop the batched leg is compared with")
     ap.add_argument("--serve-requests", type=int, default=32, help="requests per served run (0 = skip)")
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[10.0, 50.0, 200.0])
     ap.add_argument("--hbm-budget", type=float, default=200e9)
