@@ -13,14 +13,16 @@
 //   k_touch_primary    each particle inserts its primary block; one CAS per run
 //                      of equal keys in a wavefront, all runs concurrently
 //   k_touch_neighbors  each primary slot inserts its 7 +1 neighbours
-//   k_compact          occupied slots -> dense header ids (ballot + popcount,
-//                      one atomic per wavefront); zeroes the block's cell counters
-//   k_count            per-particle hash lookup (one per run of equal block keys)
+//   k_compact          occupied slots -> dense header ids (workgroup scan, one
+//                      atomic per workgroup); zeroes the block's cell counters
+//   k_count_nbr        per-particle hash lookup (one per run of equal block keys)
 //                      and per-(block, cell) rank via one aggregated atomic per
-//                      run of equal (block, cell) keys
-//   k_block_prepare    per block: wave-level exclusive scan of its 64 cell counts,
-//                      the block total, +1/-1 neighbour and previous-step ids
-//   k_block_scan       exclusive scan of the block totals (one workgroup)
+//                      run of equal (block, cell) keys; in the same launch, per
+//                      block the +1 / -1 neighbour and previous-step ids
+//   k_block_cells      per block: wave-level exclusive scan of its 64 cell
+//                      counts and the block total
+//   k_block_scan       exclusive scan of the block totals (one workgroup, or
+//                      three phases for large capacities)
 //   k_scatter          sorted slot = block start + cell offset + rank
 #include "common.h"
 
@@ -77,23 +79,23 @@ __global__ __launch_bounds__(256) void k_touch_neighbors(uint32_t *__restrict__ 
 }
 
 // Dense header ids for the occupied slots. Each workgroup takes a run of
-// COMPACT_SLOTS slots, ranks its occupied ones with a workgroup scan and reserves
-// their ids with one atomic, so the shared counter sees cap / COMPACT_SLOTS
-// atomics instead of one per wavefront.
-constexpr uint32_t COMPACT_ITEMS = 8;
-constexpr uint32_t COMPACT_SLOTS = 256 * COMPACT_ITEMS;
-
+// 256 * ITEMS slots, ranks its occupied ones with a workgroup scan and reserves
+// their ids with one atomic, so the shared counter sees cap / (256 * ITEMS)
+// atomics instead of one per wavefront (ITEMS grows with the capacity so the
+// number of atomics stays bounded while small tables keep enough workgroups).
+template <uint32_t ITEMS>
 __global__ __launch_bounds__(256) void k_compact(const uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                  uint32_t cap, uint32_t *__restrict__ hdr_key,
                                                  uint32_t *__restrict__ cell_cnt, uint32_t *__restrict__ counters) {
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t wg_base;
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
-    for (uint32_t base = blockIdx.x * COMPACT_SLOTS; base < cap; base += gridDim.x * COMPACT_SLOTS) {
-        uint32_t k[COMPACT_ITEMS];
+    constexpr uint32_t SLOTS = 256u * ITEMS;
+    for (uint32_t base = blockIdx.x * SLOTS; base < cap; base += gridDim.x * SLOTS) {
+        uint32_t k[ITEMS];
         uint32_t mine = 0;
 #pragma unroll
-        for (uint32_t i = 0; i < COMPACT_ITEMS; i++) {
+        for (uint32_t i = 0; i < ITEMS; i++) {
             const uint32_t s = base + i * 256u + tid;
             k[i] = s < cap ? keys[s] : SLOSH_NONE;
             mine += k[i] != SLOSH_NONE ? 1u : 0u;
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(256) void k_compact(const uint32_t *__restrict__ ke
         __syncthreads();
         uint32_t hid = wg_base + before + x - mine;
 #pragma unroll
-        for (uint32_t i = 0; i < COMPACT_ITEMS; i++) {
+        for (uint32_t i = 0; i < ITEMS; i++) {
             if (k[i] == SLOSH_NONE) continue;
             const uint32_t s = base + i * 256u + tid;
             vals[s] = hid;
@@ -131,12 +133,11 @@ __global__ __launch_bounds__(256) void k_compact(const uint32_t *__restrict__ ke
     }
 }
 
-__global__ __launch_bounds__(256) void k_count(const float4 *__restrict__ pos, uint32_t n, float h, uint32_t batched,
+__device__ inline void count_particle(uint32_t i, const float4 *__restrict__ pos, uint32_t n, float h, uint32_t batched,
                                                const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
                                                uint32_t cap, uint32_t *__restrict__ cell_cnt,
                                                uint32_t *__restrict__ pslot, uint32_t *__restrict__ prank,
                                                uint32_t *__restrict__ counters) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
     const uint32_t lane = lane_id();
     uint32_t cell = 0u, key = 0u;
@@ -161,50 +162,76 @@ __global__ __launch_bounds__(256) void k_count(const float4 *__restrict__ pos, u
     }
 }
 
-__global__ __launch_bounds__(256) void k_block_prepare(const uint32_t *__restrict__ cell_cnt,
-                                                       uint32_t *__restrict__ cell_start, uint32_t *__restrict__ blk_cnt,
-                                                       const uint32_t *__restrict__ hdr_key,
-                                                       uint32_t *__restrict__ nbr, const uint32_t *__restrict__ keys,
-                                                       const uint32_t *__restrict__ vals,
-                                                       const uint32_t *__restrict__ prev_keys,
-                                                       const uint32_t *__restrict__ prev_vals, uint32_t cap,
-                                                       uint32_t batched, const uint32_t *__restrict__ counters) {
-    // Per block (one wavefront): exclusive scan of the 64 cell counts (offsets
-    // relative to the block's first slot), the block total, and the neighbour
-    // table (+1 / -1 neighbours and the previous-substep header id).
+// Neighbour table of one block (one wavefront): [0..6] the +1 neighbours in
+// the reference's associated-block order (grid.slang:362-382), [7] the block's
+// id in the previous substep's table, [8..14] the -1 neighbours.
+__device__ inline void block_neighbours(uint32_t hid, uint32_t lane, const uint32_t *__restrict__ hdr_key,
+                                        uint32_t *__restrict__ nbr, const uint32_t *__restrict__ keys,
+                                        const uint32_t *__restrict__ vals, const uint32_t *__restrict__ prev_keys,
+                                        const uint32_t *__restrict__ prev_vals, uint32_t cap, uint32_t batched) {
+    if (lane >= NBR_STRIDE) return;
+    const uint32_t k = hdr_key[hid];
+    int bx, by, bz;
+    uint32_t scene;
+    unpack_key(batched, k, &bx, &by, &bz, &scene);
+    uint32_t out = SLOSH_NONE;
+    if (lane < 7) {
+        const int *o = ASSOC[lane + 1];
+        out = hash_find(keys, vals, cap, pack_key(batched, scene, bx + o[0], by + o[1], bz + o[2]));
+    } else if (lane == NBR_PREV) {
+        out = hash_find(prev_keys, prev_vals, cap, k);
+    } else if (lane >= NBR_MINUS && lane < NBR_MINUS + 7) {
+        const int *o = ASSOC[lane - NBR_MINUS + 1];
+        out = hash_find(keys, vals, cap, pack_key(batched, scene, bx - o[0], by - o[1], bz - o[2]));
+    }
+    nbr[(size_t)hid * NBR_STRIDE + lane] = out;
+}
+
+// update_block_particle_count (sort.slang:208-246) for the particles, fused with
+// update_nbh_block_ids (sort.slang:100-117) for the blocks: both only need the
+// header ids, so the first count_wgs workgroups count particles and the rest
+// fill neighbour tables (one wavefront per block).
+__global__ __launch_bounds__(256) void k_count_nbr(const float4 *__restrict__ pos, uint32_t n, float h,
+                                                   uint32_t batched, const uint32_t *__restrict__ keys,
+                                                   const uint32_t *__restrict__ vals, uint32_t cap,
+                                                   uint32_t *__restrict__ cell_cnt, uint32_t *__restrict__ pslot,
+                                                   uint32_t *__restrict__ prank, uint32_t *__restrict__ counters,
+                                                   uint32_t count_wgs, const uint32_t *__restrict__ hdr_key,
+                                                   uint32_t *__restrict__ nbr, const uint32_t *__restrict__ prev_keys,
+                                                   const uint32_t *__restrict__ prev_vals) {
+    if (blockIdx.x < count_wgs) {
+        count_particle(blockIdx.x * blockDim.x + threadIdx.x, pos, n, h, batched, keys, vals, cap, cell_cnt, pslot,
+                       prank, counters);
+        return;
+    }
+    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t stride = (gridDim.x - count_wgs) * (blockDim.x >> 6);
+    for (uint32_t hid = (blockIdx.x - count_wgs) * (blockDim.x >> 6) + wave; hid < B; hid += stride)
+        block_neighbours(hid, lane, hdr_key, nbr, keys, vals, prev_keys, prev_vals, cap, batched);
+}
+
+// Per block (one wavefront): exclusive scan of the 64 cell counts (offsets
+// relative to the block's first slot) and the block total.
+__global__ __launch_bounds__(256) void k_block_cells(const uint32_t *__restrict__ cell_cnt,
+                                                     uint32_t *__restrict__ cell_start, uint32_t *__restrict__ blk_cnt,
+                                                     const uint32_t *__restrict__ counters) {
     const uint32_t B = counters[CTR_NUM_BLOCKS];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t waves_per_grid = gridDim.x * (blockDim.x >> 6);
     for (uint32_t hid = blockIdx.x * (blockDim.x >> 6) + wave; hid < B; hid += waves_per_grid) {
-        uint32_t c = cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane];
+        const uint32_t c = cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane];
         uint32_t x = c;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
-            uint32_t y = __shfl_up(x, d);
+            const uint32_t y = __shfl_up(x, d);
             if ((int)lane >= d) x += y;
         }
         cell_start[(size_t)hid * NODES_PER_BLOCK + lane] = x - c;
         if (lane == 63) blk_cnt[hid] = x;
-        uint32_t k = hdr_key[hid];
-        int bx, by, bz;
-        uint32_t scene;
-        unpack_key(batched, k, &bx, &by, &bz, &scene);
-        uint32_t out = SLOSH_NONE;
-        if (lane < 7) {
-            const int *o = ASSOC[lane + 1];
-            out = hash_find(keys, vals, cap, pack_key(batched, scene, bx + o[0], by + o[1], bz + o[2]));
-        } else if (lane == NBR_PREV) {
-            out = hash_find(prev_keys, prev_vals, cap, k);
-        } else if (lane >= NBR_MINUS && lane < NBR_MINUS + 7) {
-            const int *o = ASSOC[lane - NBR_MINUS + 1];
-            out = hash_find(keys, vals, cap, pack_key(batched, scene, bx - o[0], by - o[1], bz - o[2]));
-        }
-        if (lane < NBR_STRIDE) nbr[(size_t)hid * NBR_STRIDE + lane] = out;
     }
 }
 
-// Exclusive scan of the block totals in one workgroup of 1024 threads (4
-// consecutive blocks per thread per pass): blk_start[b] = sum of blk_cnt[< b].
 // Exclusive scan of blk_cnt[base, base + 4096) by one workgroup of 1024 threads
 // (4 consecutive blocks per thread), written to blk_start with `carry` added;
 // returns the tile's total. Every thread of the workgroup must call it.
@@ -337,18 +364,20 @@ hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n
     }
     launch_kernel(f, k_touch_neighbors, dim3(min_u(div_up((uint64_t)cap * 8u, 256), 8192)), dim3(256), 0, stream,
                   g.keys, g.vals, cap, batched, g.counters);
-    launch_kernel(nullptr, k_compact, dim3(min_u(div_up(cap, COMPACT_SLOTS), 2048)), dim3(256), 0, stream,
-                  (const uint32_t *)g.keys, g.vals, cap,
+    if (cap <= 65536u)
+        launch_kernel(nullptr, k_compact<1>, dim3(min_u(div_up(cap, 256), 2048)), dim3(256), 0, stream,
+                      (const uint32_t *)g.keys, g.vals, cap, g.hdr_key, g.cell_cnt, g.counters);
+    else
+        launch_kernel(nullptr, k_compact<8>, dim3(min_u(div_up(cap, 2048), 2048)), dim3(256), 0, stream,
+                      (const uint32_t *)g.keys, g.vals, cap,
                   g.hdr_key, g.cell_cnt, g.counters);
-    if (n > 0)
-        launch_kernel(nullptr, k_count, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched,
-                      (const uint32_t *)g.keys, (const uint32_t *)g.vals, cap, g.cell_cnt, g.pslot, g.prank,
-                      g.counters);
-    const uint32_t blk_grid = min_u(div_up(cap, 4), 4096);
-    launch_kernel(nullptr, k_block_prepare, dim3(blk_grid), dim3(256), 0, stream, (const uint32_t *)g.cell_cnt,
-                  g.cell_start, g.blk_cnt, (const uint32_t *)g.hdr_key, g.nbr, (const uint32_t *)g.keys,
-                  (const uint32_t *)g.vals, (const uint32_t *)g.prev_keys, (const uint32_t *)g.prev_vals, cap, batched,
-                  (const uint32_t *)g.counters);
+    const uint32_t nbr_wgs = min_u(div_up(cap, 4), 2048);
+    launch_kernel(nullptr, k_count_nbr, dim3(part_grid + nbr_wgs), dim3(256), 0, stream, pos, n, h, batched,
+                  (const uint32_t *)g.keys, (const uint32_t *)g.vals, cap, g.cell_cnt, g.pslot, g.prank, g.counters,
+                  part_grid, (const uint32_t *)g.hdr_key, g.nbr, (const uint32_t *)g.prev_keys,
+                  (const uint32_t *)g.prev_vals);
+    launch_kernel(nullptr, k_block_cells, dim3(min_u(div_up(cap, 4), 4096)), dim3(256), 0, stream,
+                  (const uint32_t *)g.cell_cnt, g.cell_start, g.blk_cnt, (const uint32_t *)g.counters);
     const PassEvents *scan_ev = n > 0 ? nullptr : (ev ? &last : nullptr);
     if (cap <= 16384u) {
         launch_kernel(scan_ev, k_block_scan, dim3(1), dim3(1024), 0, stream, (const uint32_t *)g.blk_cnt, g.blk_start,
