@@ -19,10 +19,10 @@
 //                      and per-(block, cell) rank via one aggregated atomic per
 //                      run of equal (block, cell) keys; in the same launch, per
 //                      block the +1 / -1 neighbour and previous-step ids
-//   k_block_cells      per block: wave-level exclusive scan of its 64 cell
-//                      counts and the block total
 //   k_block_scan       exclusive scan of the block totals (one workgroup, or
-//                      three phases for large capacities)
+//                      three phases for large capacities; the block totals are
+//                      accumulated by k_count_nbr, one atomic per run); the
+//                      launch's other workgroups scan each block's 64 cell counts
 //   k_scatter          sorted slot = block start + cell offset + rank
 #include "common.h"
 
@@ -86,7 +86,8 @@ __global__ __launch_bounds__(256) void k_touch_neighbors(uint32_t *__restrict__ 
 template <uint32_t ITEMS>
 __global__ __launch_bounds__(256) void k_compact(const uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                  uint32_t cap, uint32_t *__restrict__ hdr_key,
-                                                 uint32_t *__restrict__ cell_cnt, uint32_t *__restrict__ counters) {
+                                                 uint32_t *__restrict__ cell_cnt, uint32_t *__restrict__ blk_cnt,
+                                                 uint32_t *__restrict__ counters) {
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t wg_base;
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
@@ -124,6 +125,7 @@ __global__ __launch_bounds__(256) void k_compact(const uint32_t *__restrict__ ke
             const uint32_t s = base + i * 256u + tid;
             vals[s] = hid;
             hdr_key[hid] = k[i];
+            blk_cnt[hid] = 0u;
             uint4 *cc = reinterpret_cast<uint4 *>(cell_cnt + (size_t)hid * NODES_PER_BLOCK);
 #pragma unroll
             for (int j = 0; j < 16; j++) cc[j] = make_uint4(0u, 0u, 0u, 0u);
@@ -137,7 +139,7 @@ __device__ inline void count_particle(uint32_t i, const float4 *__restrict__ pos
                                                const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
                                                uint32_t cap, uint32_t *__restrict__ cell_cnt,
                                                uint32_t *__restrict__ pslot, uint32_t *__restrict__ prank,
-                                               uint32_t *__restrict__ counters) {
+                                               uint32_t *__restrict__ counters, uint32_t *__restrict__ blk_cnt) {
     const bool valid = i < n;
     const uint32_t lane = lane_id();
     uint32_t cell = 0u, key = 0u;
@@ -148,6 +150,8 @@ __device__ inline void count_particle(uint32_t i, const float4 *__restrict__ pos
     if (valid && ((bheads >> lane) & 1ull)) hv = hash_find(keys, vals, cap, key);
     const uint32_t hid = __shfl(hv, run_head_of(bheads));
     const bool ok = valid && hid != SLOSH_NONE;
+    // Block totals: one atomic per run of equal block keys.
+    if (ok && ((bheads >> lane) & 1ull)) atomicAdd(&blk_cnt[hid], run_length(bheads));
     if (valid && hid == SLOSH_NONE) atomicOr(&counters[CTR_ERRORS], ERR_LOOKUP);
     const uint32_t sk = ok ? hid * NODES_PER_BLOCK + cell : SLOSH_NONE;
     // Per-(block, cell) rank: one atomic per run of equal (block, cell) keys.
@@ -196,12 +200,13 @@ __global__ __launch_bounds__(256) void k_count_nbr(const float4 *__restrict__ po
                                                    const uint32_t *__restrict__ vals, uint32_t cap,
                                                    uint32_t *__restrict__ cell_cnt, uint32_t *__restrict__ pslot,
                                                    uint32_t *__restrict__ prank, uint32_t *__restrict__ counters,
-                                                   uint32_t count_wgs, const uint32_t *__restrict__ hdr_key,
+                                                   uint32_t *__restrict__ blk_cnt, uint32_t count_wgs,
+                                                   const uint32_t *__restrict__ hdr_key,
                                                    uint32_t *__restrict__ nbr, const uint32_t *__restrict__ prev_keys,
                                                    const uint32_t *__restrict__ prev_vals) {
     if (blockIdx.x < count_wgs) {
         count_particle(blockIdx.x * blockDim.x + threadIdx.x, pos, n, h, batched, keys, vals, cap, cell_cnt, pslot,
-                       prank, counters);
+                       prank, counters, blk_cnt);
         return;
     }
     const uint32_t B = counters[CTR_NUM_BLOCKS];
@@ -212,14 +217,12 @@ __global__ __launch_bounds__(256) void k_count_nbr(const float4 *__restrict__ po
 }
 
 // Per block (one wavefront): exclusive scan of the 64 cell counts (offsets
-// relative to the block's first slot) and the block total.
-__global__ __launch_bounds__(256) void k_block_cells(const uint32_t *__restrict__ cell_cnt,
-                                                     uint32_t *__restrict__ cell_start, uint32_t *__restrict__ blk_cnt,
-                                                     const uint32_t *__restrict__ counters) {
-    const uint32_t B = counters[CTR_NUM_BLOCKS];
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t waves_per_grid = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t hid = blockIdx.x * (blockDim.x >> 6) + wave; hid < B; hid += waves_per_grid) {
+// relative to the block's first slot). Waves [w0, w0 + nw) of the grid take
+// blocks w0, w0 + nw, ...
+__device__ inline void cell_starts(const uint32_t *__restrict__ cell_cnt, uint32_t *__restrict__ cell_start,
+                                   uint32_t B, uint32_t w, uint32_t nw) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t hid = w; hid < B; hid += nw) {
         const uint32_t c = cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane];
         uint32_t x = c;
 #pragma unroll
@@ -228,7 +231,6 @@ __global__ __launch_bounds__(256) void k_block_cells(const uint32_t *__restrict_
             if ((int)lane >= d) x += y;
         }
         cell_start[(size_t)hid * NODES_PER_BLOCK + lane] = x - c;
-        if (lane == 63) blk_cnt[hid] = x;
     }
 }
 
@@ -274,11 +276,19 @@ __device__ inline uint32_t scan_block_tile(const uint32_t *__restrict__ blk_cnt,
 
 // Exclusive scan of the block totals in one workgroup: blk_start[b] = sum of
 // blk_cnt[< b] (used while the capacity is at most a few tiles).
+// Workgroups 1.. compute the per-block cell offsets meanwhile (independent work
+// in the same launch).
 __global__ __launch_bounds__(1024) void k_block_scan(const uint32_t *__restrict__ blk_cnt,
                                                      uint32_t *__restrict__ blk_start,
-                                                     uint32_t *__restrict__ counters) {
+                                                     uint32_t *__restrict__ counters,
+                                                     const uint32_t *__restrict__ cell_cnt,
+                                                     uint32_t *__restrict__ cell_start) {
     __shared__ uint32_t wsum[16];
     const uint32_t B = counters[CTR_NUM_BLOCKS];
+    if (blockIdx.x > 0) {
+        cell_starts(cell_cnt, cell_start, B, (blockIdx.x - 1) * 16u + (threadIdx.x >> 6), (gridDim.x - 1) * 16u);
+        return;
+    }
     uint32_t carry = 0;
     for (uint32_t base = 0; base < B; base += 4096u) carry += scan_block_tile(blk_cnt, blk_start, base, B, carry, wsum);
     if (threadIdx.x == 0) counters[CTR_SORTED_ALLOC] = carry;
@@ -289,10 +299,17 @@ __global__ __launch_bounds__(1024) void k_block_scan(const uint32_t *__restrict_
 __global__ __launch_bounds__(1024) void k_block_scan_tiles(const uint32_t *__restrict__ blk_cnt,
                                                            uint32_t *__restrict__ blk_start,
                                                            uint32_t *__restrict__ tile_sums,
-                                                           const uint32_t *__restrict__ counters) {
+                                                           const uint32_t *__restrict__ counters,
+                                                           uint32_t scan_wgs, const uint32_t *__restrict__ cell_cnt,
+                                                           uint32_t *__restrict__ cell_start) {
     __shared__ uint32_t wsum[16];
     const uint32_t B = counters[CTR_NUM_BLOCKS];
-    for (uint32_t t = blockIdx.x; t * 4096u < B; t += gridDim.x) {
+    if (blockIdx.x >= scan_wgs) {  // the remaining workgroups compute the cell offsets
+        cell_starts(cell_cnt, cell_start, B, (blockIdx.x - scan_wgs) * 16u + (threadIdx.x >> 6),
+                    (gridDim.x - scan_wgs) * 16u);
+        return;
+    }
+    for (uint32_t t = blockIdx.x; t * 4096u < B; t += scan_wgs) {
         const uint32_t total = scan_block_tile(blk_cnt, blk_start, t * 4096u, B, 0u, wsum);
         if (threadIdx.x == 0) tile_sums[t] = total;
     }
@@ -366,25 +383,25 @@ hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n
                   g.keys, g.vals, cap, batched, g.counters);
     if (cap <= 65536u)
         launch_kernel(nullptr, k_compact<1>, dim3(min_u(div_up(cap, 256), 2048)), dim3(256), 0, stream,
-                      (const uint32_t *)g.keys, g.vals, cap, g.hdr_key, g.cell_cnt, g.counters);
+                      (const uint32_t *)g.keys, g.vals, cap, g.hdr_key, g.cell_cnt, g.blk_cnt, g.counters);
     else
         launch_kernel(nullptr, k_compact<8>, dim3(min_u(div_up(cap, 2048), 2048)), dim3(256), 0, stream,
-                      (const uint32_t *)g.keys, g.vals, cap,
-                  g.hdr_key, g.cell_cnt, g.counters);
+                      (const uint32_t *)g.keys, g.vals, cap, g.hdr_key, g.cell_cnt, g.blk_cnt, g.counters);
     const uint32_t nbr_wgs = min_u(div_up(cap, 4), 2048);
     launch_kernel(nullptr, k_count_nbr, dim3(part_grid + nbr_wgs), dim3(256), 0, stream, pos, n, h, batched,
                   (const uint32_t *)g.keys, (const uint32_t *)g.vals, cap, g.cell_cnt, g.pslot, g.prank, g.counters,
-                  part_grid, (const uint32_t *)g.hdr_key, g.nbr, (const uint32_t *)g.prev_keys,
+                  g.blk_cnt, part_grid, (const uint32_t *)g.hdr_key, g.nbr, (const uint32_t *)g.prev_keys,
                   (const uint32_t *)g.prev_vals);
-    launch_kernel(nullptr, k_block_cells, dim3(min_u(div_up(cap, 4), 4096)), dim3(256), 0, stream,
-                  (const uint32_t *)g.cell_cnt, g.cell_start, g.blk_cnt, (const uint32_t *)g.counters);
     const PassEvents *scan_ev = n > 0 ? nullptr : (ev ? &last : nullptr);
+    const uint32_t cell_wgs = min_u(div_up(cap, 16), 1024);  // 16 blocks (wavefronts) per workgroup
     if (cap <= 16384u) {
-        launch_kernel(scan_ev, k_block_scan, dim3(1), dim3(1024), 0, stream, (const uint32_t *)g.blk_cnt, g.blk_start,
-                      g.counters);
+        launch_kernel(scan_ev, k_block_scan, dim3(1 + cell_wgs), dim3(1024), 0, stream, (const uint32_t *)g.blk_cnt,
+                      g.blk_start, g.counters, (const uint32_t *)g.cell_cnt, g.cell_start);
     } else {
-        launch_kernel(nullptr, k_block_scan_tiles, dim3(min_u(div_up(cap, 4096), 1024)), dim3(1024), 0, stream,
-                      (const uint32_t *)g.blk_cnt, g.blk_start, g.tile_sums, (const uint32_t *)g.counters);
+        const uint32_t scan_wgs = min_u(div_up(cap, 4096), 1024);
+        launch_kernel(nullptr, k_block_scan_tiles, dim3(scan_wgs + cell_wgs), dim3(1024), 0, stream,
+                      (const uint32_t *)g.blk_cnt, g.blk_start, g.tile_sums, (const uint32_t *)g.counters, scan_wgs,
+                      (const uint32_t *)g.cell_cnt, g.cell_start);
         launch_kernel(nullptr, k_block_scan_top, dim3(1), dim3(1024), 0, stream, g.tile_sums, g.counters);
         launch_kernel(scan_ev, k_block_scan_add, dim3(min_u(div_up(cap, 256), 4096)), dim3(256), 0, stream,
                       g.blk_start, (const uint32_t *)g.tile_sums, (const uint32_t *)g.counters);
