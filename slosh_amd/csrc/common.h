@@ -147,20 +147,54 @@ __device__ inline uint32_t lane_id() { return __lane_id(); }
 __device__ inline uint64_t lanemask_lt() { return __lanemask_lt(); }
 
 // Open-addressing insertion with a strong CAS (insertion_index, grid.slang:118-174).
-// `primary` marks the slot as a primary block for the neighbour expansion.
-__device__ inline void hash_insert(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals, uint32_t cap,
-                                   uint32_t key, bool primary, uint32_t *__restrict__ counters) {
+// A plain load first: most insertions find their key already present (many
+// runs of a wavefront, and many wavefronts, touch the same block), which then
+// costs no atomic. A stale NONE only leads to the CAS, which sees the truth.
+__device__ inline void hash_insert(uint32_t *__restrict__ keys, uint32_t cap, uint32_t key,
+                                   uint32_t *__restrict__ counters) {
     uint32_t slot = hash_key(key) & (cap - 1u);
     for (uint32_t k = 0; k < cap; k++) {
-        uint32_t old = atomicCAS(&keys[slot], SLOSH_NONE, key);
-        if (old == SLOSH_NONE) {
-            if (primary) vals[slot] = 1u;
-            return;
+        const uint32_t cur = keys[slot];
+        if (cur == key) return;
+        if (cur == SLOSH_NONE) {
+            const uint32_t old = atomicCAS(&keys[slot], SLOSH_NONE, key);
+            if (old == SLOSH_NONE || old == key) return;
         }
-        if (old == key) return;
         slot = (slot + 1u) & (cap - 1u);
     }
     atomicOr(&counters[CTR_ERRORS], ERR_HASH_FULL);
+}
+
+// Associated blocks of a block (grid.slang:362-382): offset 0 is the block
+// itself, 1..7 its "+1" neighbours.
+__device__ constexpr int BLOCK_ASSOC[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1},
+                                              {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+
+// touch_primary_blocks + touch_neighbor_blocks (sort.slang:50-92) for the keys
+// of the lanes in `heads` (a wavefront-uniform lane mask): every head's block
+// and its 7 "+1" neighbours are inserted, with the 64 lanes spread over
+// (head, offset) pairs, eight heads per round. Every lane of the wavefront must
+// call it.
+__device__ inline void insert_blocks(uint32_t *__restrict__ keys, uint32_t cap, uint32_t batched, uint32_t key,
+                                     uint64_t heads, uint32_t *__restrict__ counters) {
+    const uint32_t lane = __lane_id();
+    const uint32_t r = lane >> 3, o = lane & 7u;
+    while (heads) {
+        uint64_t m = heads;
+        for (uint32_t k = 0; k < r && m; k++) m &= m - 1ull;
+        const int src = m ? __ffsll((unsigned long long)m) - 1 : 0;
+        const uint32_t hk = __shfl(key, src);
+        if (m) {
+            int x, y, z;
+            uint32_t scene;
+            unpack_key(batched, hk, &x, &y, &z, &scene);
+            hash_insert(keys, cap,
+                        pack_key(batched, scene, x + BLOCK_ASSOC[o][0], y + BLOCK_ASSOC[o][1], z + BLOCK_ASSOC[o][2]),
+                        counters);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) heads &= heads ? heads - 1ull : 0ull;
+    }
 }
 
 // Primary-block key of a particle (x, y, z, w = scene << 1 | enabled bits).

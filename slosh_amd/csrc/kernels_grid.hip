@@ -10,9 +10,10 @@
 // Kernel sequence (one launch each, all grid-stride / upper-bound grids so the
 // sequence is graph-capturable without reading counts back):
 //   k_reset_table      hash slots <- NONE, per-step counters <- 0
-//   k_touch_primary    each particle inserts its primary block; one CAS per run
-//                      of equal keys in a wavefront, all runs concurrently
-//   k_touch_neighbors  each primary slot inserts its 7 +1 neighbours
+//   k_touch_primary    each particle's primary block and its 7 +1 neighbours are
+//                      inserted, once per run of equal keys in a wavefront (also
+//                      done by the previous substep's fused G2P kernel, which
+//                      then replaces this kernel and k_reset_table)
 //   k_compact          occupied slots -> dense header ids (workgroup scan, one
 //                      atomic per workgroup); zeroes the block's cell counters
 //   k_count_nbr        per-particle hash lookup (one per run of equal block keys)
@@ -52,30 +53,9 @@ __global__ __launch_bounds__(256) void k_touch_primary(const float4 *__restrict_
     const bool valid = i < n;
     uint32_t key = 0u, cell;
     if (valid) key = particle_key(pos[i], h, batched, &cell);
-    // One insertion per run of equal keys; all runs of the wavefront insert concurrently.
-    const uint64_t heads = run_heads(key, valid);
-    if (valid && ((heads >> lane_id()) & 1ull)) hash_insert(keys, vals, cap, key, true, counters);
-}
-
-__global__ __launch_bounds__(256) void k_touch_neighbors(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                         uint32_t cap, uint32_t batched,
-                                                         uint32_t *__restrict__ counters) {
-    // touch_neighbor_blocks (sort.slang:75-92): only slots flagged primary by
-    // k_touch_primary expand; slots this kernel fills keep flag 0. One lane per
-    // (slot, neighbour offset) so the 7 insertions of a block run in parallel.
-    const uint64_t total = (uint64_t)cap * 8u;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
-        const uint32_t s = (uint32_t)(t >> 3), o = (uint32_t)(t & 7u);
-        if (o == 0) continue;
-        uint32_t k = keys[s];
-        if (k == SLOSH_NONE || vals[s] != 1u) continue;
-        int x, y, z;
-        uint32_t scene;
-        unpack_key(batched, k, &x, &y, &z, &scene);
-        hash_insert(keys, vals, cap, pack_key(batched, scene, x + ASSOC[o][0], y + ASSOC[o][1], z + ASSOC[o][2]),
-                    false, counters);
-    }
+    // One insertion round per run of equal keys: the block and its 7 +1 neighbours.
+    const uint64_t heads = run_heads(key, valid) & __ballot(valid);
+    insert_blocks(keys, cap, batched, key, heads, counters);
 }
 
 // Dense header ids for the occupied slots. Each workgroup takes a run of
@@ -379,13 +359,11 @@ hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n
             launch_kernel(f, k_touch_primary, dim3(part_grid), dim3(256), 0, stream, pos, n, h, batched, g.keys,
                           g.vals, cap, g.counters);
     }
-    launch_kernel(f, k_touch_neighbors, dim3(min_u(div_up((uint64_t)cap * 8u, 256), 8192)), dim3(256), 0, stream,
-                  g.keys, g.vals, cap, batched, g.counters);
     if (cap <= 65536u)
-        launch_kernel(nullptr, k_compact<1>, dim3(min_u(div_up(cap, 256), 2048)), dim3(256), 0, stream,
+        launch_kernel(f, k_compact<1>, dim3(min_u(div_up(cap, 256), 2048)), dim3(256), 0, stream,
                       (const uint32_t *)g.keys, g.vals, cap, g.hdr_key, g.cell_cnt, g.blk_cnt, g.counters);
     else
-        launch_kernel(nullptr, k_compact<8>, dim3(min_u(div_up(cap, 2048), 2048)), dim3(256), 0, stream,
+        launch_kernel(f, k_compact<8>, dim3(min_u(div_up(cap, 2048), 2048)), dim3(256), 0, stream,
                       (const uint32_t *)g.keys, g.vals, cap, g.hdr_key, g.cell_cnt, g.blk_cnt, g.counters);
     const uint32_t nbr_wgs = min_u(div_up(cap, 4), 2048);
     launch_kernel(nullptr, k_count_nbr, dim3(part_grid + nbr_wgs), dim3(256), 0, stream, pos, n, h, batched,

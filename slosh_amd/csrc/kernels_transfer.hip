@@ -1054,7 +1054,8 @@ __global__ __launch_bounds__(NT, F64 ? 3 : 1) void k_g2p_update(StepParams P, PB
                                                     const uint32_t *__restrict__ blk_start,
                                                     const uint32_t *__restrict__ blk_cnt,
                                                     const uint32_t *__restrict__ nbr, const float4 *__restrict__ nodes,
-                                                    const uint32_t *__restrict__ counters, NextTable nt, Bodies bodies) {
+                                                    const uint32_t *__restrict__ counters, NextTable nt, Bodies bodies,
+                                                    const uint32_t *__restrict__ hdr_key) {
     __shared__ float4 sh[TILE_NODES];
     // integrate_bodies: reads no particle or grid state (the impulses it applies
     // are only written by the host), so it runs alongside the particle update.
@@ -1063,6 +1064,8 @@ __global__ __launch_bounds__(NT, F64 ? 3 : 1) void k_g2p_update(StepParams P, PB
     for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
         const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
         if (cnt == 0) continue;
+        const uint32_t block_key = hdr_key[hid];
+        bool stays = false;
         load_halo(sh, nodes, nbr, hid);
         __syncthreads();
         // Every lane of the workgroup runs the same number of iterations so the
@@ -1118,11 +1121,24 @@ __global__ __launch_bounds__(NT, F64 ? 3 : 1) void k_g2p_update(StepParams P, PB
             }
             }
             if (nt.prime) {
-                // touch_primary_blocks of the next substep (sort.slang:50-67), one
-                // insertion per run of equal keys.
-                const uint64_t heads = run_heads(next_key, valid);
-                if (valid && ((heads >> lane_id()) & 1ull)) hash_insert(nt.keys, nt.vals, P.cap, next_key, true, nt.counters);
+                // touch_primary_blocks and touch_neighbor_blocks of the next
+                // substep (sort.slang:50-92). Particles that stay in this block
+                // only flag it (its 8 blocks are inserted once below); the others
+                // insert theirs, one round per run of equal keys.
+                const bool mover = valid && next_key != block_key;
+                stays |= valid && !mover;
+                const uint64_t heads = run_heads(next_key, mover) & __ballot(mover);
+                insert_blocks(nt.keys, P.cap, P.batched, next_key, heads, nt.counters);
             }
+        }
+        if (nt.prime && __syncthreads_or(stays) && threadIdx.x < 8) {
+            int x, y, z;
+            uint32_t scene;
+            unpack_key(P.batched, block_key, &x, &y, &z, &scene);
+            hash_insert(nt.keys, P.cap,
+                        pack_key(P.batched, scene, x + BLOCK_ASSOC[threadIdx.x][0], y + BLOCK_ASSOC[threadIdx.x][1],
+                                 z + BLOCK_ASSOC[threadIdx.x][2]),
+                        nt.counters);
         }
         __syncthreads();
     }
@@ -1365,7 +1381,7 @@ hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, con
                                      : (narrow ? k_g2p_update<15u, 128, false> : k_g2p_update<15u, 256, false>));
     launch_kernel(ev, kern, dim3(block_grid(P, 1)), dim3(narrow ? 128 : 256), 0, stream, P, to_pbuf(in),
                   to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes,
-                  t.counters, nt, b);
+                  t.counters, nt, b, t.hdr_key);
     return hipGetLastError();
 }
 
