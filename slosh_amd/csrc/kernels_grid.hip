@@ -184,15 +184,19 @@ __global__ __launch_bounds__(256) void k_count_nbr(const float4 *__restrict__ po
                                                    const uint32_t *__restrict__ hdr_key,
                                                    uint32_t *__restrict__ nbr, const uint32_t *__restrict__ prev_keys,
                                                    const uint32_t *__restrict__ prev_vals) {
-    if (blockIdx.x < count_wgs) {
-        count_particle(blockIdx.x * blockDim.x + threadIdx.x, pos, n, h, batched, keys, vals, cap, cell_cnt, pslot,
-                       prank, counters, blk_cnt);
+    // The neighbour workgroups come first in dispatch order: each walks a long
+    // chain of dependent hash probes, so they start while the counting
+    // workgroups (short, independent) fill the rest of the machine.
+    const uint32_t nbr_wgs = gridDim.x - count_wgs;
+    if (blockIdx.x >= nbr_wgs) {
+        count_particle((blockIdx.x - nbr_wgs) * blockDim.x + threadIdx.x, pos, n, h, batched, keys, vals, cap,
+                       cell_cnt, pslot, prank, counters, blk_cnt);
         return;
     }
     const uint32_t B = counters[CTR_NUM_BLOCKS];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t stride = (gridDim.x - count_wgs) * (blockDim.x >> 6);
-    for (uint32_t hid = (blockIdx.x - count_wgs) * (blockDim.x >> 6) + wave; hid < B; hid += stride)
+    const uint32_t stride = nbr_wgs * (blockDim.x >> 6);
+    for (uint32_t hid = blockIdx.x * (blockDim.x >> 6) + wave; hid < B; hid += stride)
         block_neighbours(hid, lane, hdr_key, nbr, keys, vals, prev_keys, prev_vals, cap, batched);
 }
 
