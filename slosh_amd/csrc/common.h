@@ -37,6 +37,8 @@ struct StepParams {
     float dt;
     float h;      // cell width
     float inv_h;  // 1.0f / h, as p2g_scatter_style.slang:54
+    float h_dt;   // h / dt (FP32 division, as the velocity clamp computes it)
+    float inv_d;  // 4.0f / (h * h), the APIC inverse inertia factor
     uint32_t cap; // hash capacity (power of two) == max blocks
     uint32_t n;   // particle count
     uint32_t batched;     // 0: pack_key of grid.slang:83-90; 1: scene << 18 | 6-bit x, y, z (offset 16)
@@ -97,11 +99,22 @@ __device__ __host__ inline uint32_t hash_key(uint32_t k) {
 // Associated cell (round(x / h) - 1, half-to-even as WGSL round) and the block
 // it belongs to: block_associated_to_point (grid.slang:346-355) and
 // associated_cell_index_in_block_off_by_one (particle.slang:79-87).
+// x / h. For a power-of-two h (normal, with a normal reciprocal) the product
+// with the exact reciprocal 2^-e is the same number as the quotient: both are
+// exact scalings, or round the same exact value. The test is uniform (h is a
+// kernel argument), so the other path costs one scalar branch.
+__device__ inline float div_h(float x, float h) {
+    const uint32_t hb = __float_as_uint(h);
+    if ((hb & 0x807fffffu) == 0u && hb - 0x00800000u < 0x7e000000u)
+        return x * __uint_as_float(0x7f000000u - (hb & 0x7f800000u));
+    return x / h;
+}
+
 __device__ inline void assoc_block_cell(float h, float px, float py, float pz, int *bx, int *by, int *bz,
                                         uint32_t *cell) {
-    float ax = rintf(px / h) - 1.0f;
-    float ay = rintf(py / h) - 1.0f;
-    float az = rintf(pz / h) - 1.0f;
+    float ax = rintf(div_h(px, h)) - 1.0f;
+    float ay = rintf(div_h(py, h)) - 1.0f;
+    float az = rintf(div_h(pz, h)) - 1.0f;
     float fx = floorf(ax / 4.0f), fy = floorf(ay / 4.0f), fz = floorf(az / 4.0f);
     *bx = (int)fx;
     *by = (int)fy;

@@ -580,8 +580,8 @@ struct PStatic {
 // With dpt_i = ref + s_i h (s_i in {0,1,2}^3 the node's stencil shift), the sums
 // are formed from V = sum w v_i and S_a = sum w s_i,a v_i (a = x, y, z):
 // sum w v_i dpt_i,a = V ref_a + h S_a.
-__device__ inline void g2p_particle(const float4 *__restrict__ sh, float h, float3 p, bool enabled, float3 *vel,
-                                    float *vg, float *vgd_out) {
+__device__ inline void g2p_particle(const float4 *__restrict__ sh, float h, float inv_d, float3 p, bool enabled,
+                                    float3 *vel, float *vg, float *vgd_out) {
     float V[3] = {0.0f, 0.0f, 0.0f}, S[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
     float ref[3] = {0.0f, 0.0f, 0.0f};
     if (enabled) {
@@ -590,33 +590,50 @@ __device__ inline void g2p_particle(const float4 *__restrict__ sh, float h, floa
         uint32_t lc[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            float ac = rintf(pp[i] / h) - 1.0f;
+            float ac = rintf(div_h(pp[i], h)) - 1.0f;
             ref[i] = ac * h - pp[i];
-            k_eval_all(-ref[i] / h, w[i]);
+            k_eval_all(div_h(-ref[i], h), w[i]);
             float ab = floorf(ac / 4.0f) * 4.0f;
             lc[i] = (uint32_t)(ac - ab);
         }
         const uint32_t packed = lc[0] + lc[1] * 6u + lc[2] * 36u;
+        // V = sum_n w_n v_n and S[d] = sum_n s_d(n) w_n v_n (s_d = the node's stencil
+        // offset 0..2 along d), with the separable weight w_n = wx wy wz summed
+        // one axis at a time: rows along x first, then y within each z slab, then z.
+        const float wx2 = 2.0f * w[0][2], wy2 = 2.0f * w[1][2], wz2 = 2.0f * w[2][2];
 #pragma unroll
-        for (int i = 0; i < 27; i++) {
-            const uint32_t sx = NBH_C[i][0], sy = NBH_C[i][1], sz = NBH_C[i][2];
-            const float4 cd = sh[packed + sx + sy * 6u + sz * 36u];
-            const float weight = w[0][sx] * w[1][sy] * w[2][sz];
-            const float wv[3] = {cd.x * weight, cd.y * weight, cd.z * weight};
+        for (int sz = 0; sz < 3; sz++) {
+            float bz[3] = {0.0f, 0.0f, 0.0f}, myz[3] = {0.0f, 0.0f, 0.0f}, mxz[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int sy = 0; sy < 3; sy++) {
+                const uint32_t base = packed + (uint32_t)sy * 6u + (uint32_t)sz * 36u;
+                const float4 c0 = sh[base], c1 = sh[base + 1u], c2 = sh[base + 2u];
+                const float r[3] = {w[0][0] * c0.x + w[0][1] * c1.x + w[0][2] * c2.x,
+                                    w[0][0] * c0.y + w[0][1] * c1.y + w[0][2] * c2.y,
+                                    w[0][0] * c0.z + w[0][1] * c1.z + w[0][2] * c2.z};
+                const float mx[3] = {w[0][1] * c1.x + wx2 * c2.x, w[0][1] * c1.y + wx2 * c2.y,
+                                     w[0][1] * c1.z + wx2 * c2.z};
+                const float wy = w[1][sy];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    bz[c] += wy * r[c];
+                    mxz[c] += wy * mx[c];
+                    if (sy == 1) myz[c] += wy * r[c];
+                    if (sy == 2) myz[c] += wy2 * r[c];
+                }
+            }
+            const float wz = w[2][sz];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                V[c] += wv[c];
-                if (sx == 1) S[0][c] += wv[c];
-                if (sx == 2) S[0][c] += 2.0f * wv[c];
-                if (sy == 1) S[1][c] += wv[c];
-                if (sy == 2) S[1][c] += 2.0f * wv[c];
-                if (sz == 1) S[2][c] += wv[c];
-                if (sz == 2) S[2][c] += 2.0f * wv[c];
+                V[c] += wz * bz[c];
+                S[0][c] += wz * mxz[c];
+                S[1][c] += wz * myz[c];
+                if (sz == 1) S[2][c] += wz * bz[c];
+                if (sz == 2) S[2][c] += wz2 * bz[c];
             }
         }
     }
     *vel = make_float3(V[0], V[1], V[2]);
-    const float inv_d = 4.0f / (h * h);
     float vgd = 0.0f;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
@@ -641,12 +658,12 @@ __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 
 #pragma clang fp contract(off)
     const float dt = P.dt, h = P.h;
     float vl = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
-    if (vl > h / dt) {
+    if (vl > P.h_dt) {
         float l2 = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
         v = make_float3(v.x / l2 * h / dt, v.y / l2 * h / dt, v.z / l2 * h / dt);
     }
     float damp = 1.0f + prop.y * dt;
-    v = make_float3(v.x / damp, v.y / damp, v.z / damp);
+    if (damp != 1.0f) v = make_float3(v.x / damp, v.y / damp, v.z / damp);
     bool fixed = __float_as_uint(prop.z) != 0u;
     if (fixed) v = make_float3(0.0f, 0.0f, 0.0f);
     float3 np = make_float3(pos.x + v.x * dt, pos.y + v.y * dt, pos.z + v.z * dt);
@@ -681,8 +698,7 @@ __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 
 #pragma unroll
         for (int k = 0; k < 9; k++) stress[k] = 0.0f;
     }
-    float inv_d = 4.0f / (h * h);
-    float sk = prop.x * inv_d * dt;
+    float sk = prop.x * P.inv_d * dt;
     bool ok = np.x == np.x && np.y == np.y && np.z == np.z && det3(F) > 0.0f;
     if (ok) {
         pos = np;
@@ -931,7 +947,7 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
         // update_single_cell, grid_update.slang:154-163
         const float mass = acc.w;
         const float inv_mass = mass > 0.0f ? 1.0f / mass : 0.0f;
-        const float lim = h / dt;
+        const float lim = P.h_dt;
         float3 v;
         v.x = fminf(fmaxf((acc.x + mass * P.gravity[0] * dt) * inv_mass, -lim), lim);
         v.y = fminf(fmaxf((acc.y + mass * P.gravity[1] * dt) * inv_mass, -lim), lim);
@@ -1084,7 +1100,7 @@ __global__ __launch_bounds__(NT, F64 ? 3 : 1) void k_g2p_update(StepParams P, PB
             float3 v;
             float aff[9], F[9];
             float vgd;
-            g2p_particle(sh, P.h, pos, enabled != 0u, &v, aff, &vgd);
+            g2p_particle(sh, P.h, P.inv_d, pos, enabled != 0u, &v, aff, &vgd);
             float mass = in.vm[p].w;
 #pragma unroll
             for (int k = 0; k < 9; k++) F[k] = in.F[(size_t)k * pcap + p];
@@ -1208,7 +1224,8 @@ __global__ __launch_bounds__(256) void k_g2p(StepParams P, PBuf buf, uint32_t pc
             float3 v;
             float g[9];
             float vgd;
-            g2p_particle(sh, P.h, make_float3(ps.x, ps.y, ps.z), (__float_as_uint(ps.w) & 1u) != 0u, &v, g, &vgd);
+            g2p_particle(sh, P.h, P.inv_d, make_float3(ps.x, ps.y, ps.z), (__float_as_uint(ps.w) & 1u) != 0u, &v, g,
+                         &vgd);
             float4 o = buf.vm[p];
             buf.vm[p] = make_float4(v.x, v.y, v.z, o.w);
 #pragma unroll

@@ -551,6 +551,8 @@ static StepParams make_params(const slosh_data *d) {
     P.dt = d->params.dt;
     P.h = d->h;
     P.inv_h = 1.0f / d->h;
+    P.h_dt = d->h / d->params.dt;
+    P.inv_d = 4.0f / (d->h * d->h);
     P.cap = d->cap;
     P.n = (uint32_t)d->n;
     P.batched = d->batched;

