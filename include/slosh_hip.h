@@ -293,7 +293,7 @@ int slosh_device_view_get(slosh_data *data, slosh_device_view *out);
 
 /* ---- Launch shape (no reference counterpart: the wgpu pipeline fixes its
  * workgroup sizes in the shaders). p2g_chunk: particles staged in LDS per P2G
- * round (256, 512, 1024); g2p_threads: threads per G2P/particle-update
+ * round (192, 256, 384, 512); g2p_threads: threads per G2P/particle-update
  * workgroup (128, 256); block_wgs: workgroups of the per-block kernels, which
  * loop over the active blocks (64..65536). Results do not depend on it beyond
  * summation order within a cell. slosh_autotune times `trial_steps` substeps of

@@ -722,11 +722,15 @@ __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 
 // P2G for one block per workgroup of 3 waves.
 // Phase 1 (per chunk of P2G_CHUNK sorted particles of the block): each thread
 // loads one particle with coalesced reads (particle storage follows the block
-// order of the previous substep) and stages position, mass, momentum and the
-// APIC matrix in LDS.
+// order of the previous substep) and stages in LDS what every node of its
+// stencil shares: mass, the APIC momentum at the stencil's first node
+// A0 = m v + C^T (x_0 - x_p), the particle's offset f from that node in cells
+// (the kernel weights follow from it in a few operations, cheaper than
+// staging them: LDS per particle bounds how many workgroups share a CU), and C.
 // Phase 2: lane = associated cell of the block, wave = z offset kz of the
 // stencil; each lane accumulates, over the particles of its cell in the chunk,
-// the contributions to the 9 nodes (cell + (kx, ky, kz)) in 36 registers.
+// the contributions w_n (A0 + C^T (n h)) to the 9 nodes n = (kx, ky, kz) in 36
+// registers.
 // Phase 3: the waves merge their registers into the 6x6x6 LDS tile one after
 // the other, in 9 rounds in which the 64 lanes of a wave touch 64 distinct nodes
 // (plain read-modify-write; no LDS or global atomics). The summation order is
@@ -757,7 +761,9 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
     __shared__ float4 tile[TILE_NODES];
     constexpr int P2G_LDS = P2G_CHUNK + P2G_CHUNK / 8;
     constexpr int R = (P2G_CHUNK + 191) / 192;
-    __shared__ float sp[16][P2G_LDS];  // x, y, z, m, mom x, y, z, C[9]
+    // m, A0[3], f[3], C[9]
+    constexpr int SP_M = 0, SP_A = 1, SP_F = 4, SP_C = 7;
+    __shared__ float sp[16][P2G_LDS];
     if (nt.prime) {
         for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < P.cap; s += gridDim.x * blockDim.x) {
             nt.keys[s] = SLOSH_NONE;
@@ -772,7 +778,8 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
     const uint32_t kz = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
     const uint32_t tbase = lx + 6 * ly + 36 * (lz + (int)kz);
-    const float h = P.h, inv_h = P.inv_h;
+    const float h = P.h;
+    const float kzh = (float)kz * h;
     for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
         const uint32_t b0 = blk_start[hid], nb = blk_cnt[hid];
         // A block without particles (a "+1" neighbour only) has an all-zero tile:
@@ -784,13 +791,6 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
         unpack_key(P.batched, hdr_key[hid], &bx, &by, &bz, &scene);
         const uint32_t cs = cell_start[(size_t)hid * NODES_PER_BLOCK + lane];
         const uint32_t ce = cs + cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane];
-        float nx[3], ny[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            nx[k] = (float)(bx * 4 + lx + k) * h;
-            ny[k] = (float)(by * 4 + ly + k) * h;
-        }
-        const float nz = (float)(bz * 4 + lz + (int)kz) * h;
         float acc[9][4];
 #pragma unroll
         for (int k = 0; k < 9; k++) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.0f;
@@ -818,58 +818,67 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
             for (int r = 0; r < R; r++) {
                 if (pp[r] == SLOSH_NONE) continue;
                 const uint32_t p = pp[r];
-                float mx = vv[r].x * vv[r].w, my = vv[r].y * vv[r].w, mz = vv[r].z * vv[r].w;
+                float mom[3] = {vv[r].x * vv[r].w, vv[r].y * vv[r].w, vv[r].z * vv[r].w};
                 if (P.has_force) {
-                    mx += fdt[p];
-                    my += fdt[(size_t)pcap + p];
-                    mz += fdt[2 * (size_t)pcap + p];
-                }
-                const uint32_t q = p2g_slot(threadIdx.x + 192u * r);
-                sp[0][q] = ps[r].x;
-                sp[1][q] = ps[r].y;
-                sp[2][q] = ps[r].z;
-                sp[3][q] = vv[r].w;
-                sp[4][q] = mx;
-                sp[5][q] = my;
-                sp[6][q] = mz;
 #pragma unroll
-                for (int k = 0; k < 9; k++) sp[7 + k][q] = cc[r][k];
+                    for (int i = 0; i < 3; i++) mom[i] += fdt[(size_t)i * pcap + p];
+                }
+                // The associated cell (the block and cell the sort filed the
+                // particle under): node 0 of the stencil sits at d0 = x_0 - x_p.
+                const float xp[3] = {ps[r].x, ps[r].y, ps[r].z};
+                float d0[3], f[3];
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const float ac = rintf(div_h(xp[i], h)) - 1.0f;
+                    d0[i] = ac * h - xp[i];
+                    f[i] = div_h(-d0[i], h);
+                }
+                const float *c = cc[r];
+                const uint32_t q = p2g_slot(threadIdx.x + 192u * r);
+                sp[SP_M][q] = vv[r].w;
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    sp[SP_A + i][q] = mom[i] + (d0[0] * c[i] + d0[1] * c[3 + i] + d0[2] * c[6 + i]);
+                    sp[SP_F + i][q] = f[i];
+                }
+#pragma unroll
+                for (int k = 0; k < 9; k++) sp[SP_C + k][q] = c[k];
             }
             __syncthreads();
             const uint32_t j0 = cs > chunk ? cs : chunk;
             const uint32_t j1 = ce < chunk + P2G_CHUNK ? ce : chunk + P2G_CHUNK;
             for (uint32_t j = j0; j < j1; j++) {
                 const uint32_t q = p2g_slot(j - chunk);
-                const float px = sp[0][q], py = sp[1][q], pz = sp[2][q], m = sp[3][q];
-                float c[9];
+                const float m = sp[SP_M][q];
+                const float wz = k_eval(sp[SP_F + 2][q] - (float)kz);
+                float c[9], wx[3], wy[3], wyz[3], az[3];
+                k_eval_all(sp[SP_F][q], wx);
+                k_eval_all(sp[SP_F + 1][q], wy);
 #pragma unroll
-                for (int k = 0; k < 9; k++) c[k] = sp[7 + k][q];
-                float dx[3], dy[3], wx[3], wy[3];
+                for (int k = 0; k < 9; k++) c[k] = sp[SP_C + k][q];
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    dx[k] = nx[k] - px;
-                    dy[k] = ny[k] - py;
-                    wx[k] = k_eval(dx[k] * inv_h);
-                    wy[k] = k_eval(dy[k] * inv_h);
+                for (int i = 0; i < 3; i++) {
+                    wyz[i] = wy[i] * wz;
+                    // A0 + C^T (0, 0, kz h): shared by the 9 nodes of the slab.
+                    az[i] = sp[SP_A + i][q] + kzh * c[6 + i];
                 }
-                const float dz = nz - pz;
-                const float wz = k_eval(dz * inv_h);
-                // mul(dpt, affine) + momentum: the z term is shared by the 9 nodes of the slab.
-                const float bx_ = dz * c[6] + sp[4][q], by_ = dz * c[7] + sp[5][q], bz_ = dz * c[8] + sp[6][q];
 #pragma unroll
-                for (int ky = 0; ky < 3; ky++)
+                for (int ky = 0; ky < 3; ky++) {
+                    float ay[3];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) ay[i] = ky == 0 ? az[i] : az[i] + (float)ky * h * c[3 + i];
 #pragma unroll
                     for (int kx = 0; kx < 3; kx++) {
                         const int k = kx + 3 * ky;
-                        const float w = wx[kx] * wy[ky] * wz;
-                        const float ax = dx[kx] * c[0] + dy[ky] * c[3] + bx_;
-                        const float ay = dx[kx] * c[1] + dy[ky] * c[4] + by_;
-                        const float az = dx[kx] * c[2] + dy[ky] * c[5] + bz_;
-                        acc[k][0] += ax * w;
-                        acc[k][1] += ay * w;
-                        acc[k][2] += az * w;
-                        acc[k][3] += m * w;
+                        const float wgt = wx[kx] * wyz[ky];
+#pragma unroll
+                        for (int i = 0; i < 3; i++) {
+                            const float a = kx == 0 ? ay[i] : ay[i] + (float)kx * h * c[i];
+                            acc[k][i] += a * wgt;
+                        }
+                        acc[k][3] += m * wgt;
                     }
+                }
             }
         }
         __syncthreads();
@@ -1350,7 +1359,10 @@ hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t p
     nt.vals = next.vals;
     nt.counters = next.counters;
     nt.prime = next.prime;
-    auto kern = P.p2g_chunk == 256u ? k_p2g<256> : (P.p2g_chunk == 1024u ? k_p2g<1024> : k_p2g<512>);
+    auto kern = P.p2g_chunk == 192u ? k_p2g<192>
+                : P.p2g_chunk == 256u ? k_p2g<256>
+                : P.p2g_chunk == 384u ? k_p2g<384>
+                                      : k_p2g<512>;
     launch_kernel(ev, kern, dim3(block_grid(P, 1)), dim3(192), 0, stream, P, nt, (const float4 *)pb.pos,
                   (const float4 *)pb.vm, (const float *)pb.C, (const float *)t.fdt, pcap, t.perm, t.cell_cnt,
                   t.cell_start, t.blk_start, t.blk_cnt, t.hdr_key, t.tiles, t.counters);

@@ -1664,7 +1664,7 @@ int slosh_read_scene(slosh_data *d, slosh_stream stream, uint32_t scene_id, floa
 
 // ---- launch-shape tuning ------------------------------------------------------
 static bool tuning_valid(const slosh_tuning &t) {
-    return (t.p2g_chunk == 256u || t.p2g_chunk == 512u || t.p2g_chunk == 1024u) &&
+    return (t.p2g_chunk == 192u || t.p2g_chunk == 256u || t.p2g_chunk == 384u || t.p2g_chunk == 512u) &&
            (t.g2p_threads == 128u || t.g2p_threads == 256u) && t.block_wgs >= 64u && t.block_wgs <= 65536u;
 }
 
@@ -1731,7 +1731,7 @@ int slosh_autotune(slosh_pipeline *pl, slosh_data *d, slosh_stream stream, uint3
     int rc = run_step(pl, c, st, nullptr, nullptr, nullptr);
     if (rc == SLOSH_OK) rc = time_tuning(pl, c, st, best, steps, &best_ms);
     // Coordinate search: P2G staging chunk, G2P workgroup size, persistent width.
-    const uint32_t chunks[] = {256u, 512u, 1024u};
+    const uint32_t chunks[] = {192u, 256u, 384u, 512u};
     const uint32_t threads[] = {128u, 256u};
     const uint32_t widths[] = {2048u, 4096u, 8192u, 16384u, 32768u};
     auto consider = [&](slosh_tuning t) {

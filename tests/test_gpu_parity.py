@@ -125,11 +125,12 @@ def test_parity_mixed(gpu_pipeline, steps):
     assert np.allclose(gv["linear"], ov["linear"], rtol=1e-5, atol=1e-6)
 
 
-# Every launch shape computes the same substep (P2G staging rounds of 256 / 512 /
-# 1024 particles, G2P workgroups of 128 / 256, persistent widths).
+# Every launch shape computes the same substep (P2G staging rounds of 192 / 256 /
+# 384 / 512 particles, G2P workgroups of 128 / 256, persistent widths).
 SHAPES = [{"p2g_chunk": 512, "g2p_threads": 256, "block_wgs": 4096},
           {"p2g_chunk": 256, "g2p_threads": 128, "block_wgs": 2048},
-          {"p2g_chunk": 1024, "g2p_threads": 256, "block_wgs": 16384}]
+          {"p2g_chunk": 384, "g2p_threads": 256, "block_wgs": 16384},
+          {"p2g_chunk": 192, "g2p_threads": 128, "block_wgs": 8192}]
 
 
 @pytest.mark.parametrize("shape", range(len(SHAPES)))
