@@ -88,11 +88,22 @@ __device__ __host__ inline void unpack_key(uint32_t batched, uint32_t k, int *x,
     }
 }
 
-// Murmur3-style mixing, grid.slang:93-100
+// Slot hash: the Murmur3 key mixing of grid.slang:93-100 followed by Murmur3's
+// 32-bit finalizer (fmix32). Without the finalizer the low bits of the mix keep
+// the lattice structure of block keys: on the 1M-particle dam break (3400
+// blocks in 16384 slots) linear probing then needs 3.5 probes per lookup on
+// average and up to 22, against 1.2 and at most 7 with it. Which slots the keys
+// occupy changes only the table's internal order (header ids); the block set,
+// the per-block counts and every particle's result do not depend on it.
 __device__ __host__ inline uint32_t hash_key(uint32_t k) {
     k *= 0xcc9e2d51u;
     k = (k << 15) | (k >> 17);
     k *= 0x1b873593u;
+    k ^= k >> 16;
+    k *= 0x85ebca6bu;
+    k ^= k >> 13;
+    k *= 0xc2b2ae35u;
+    k ^= k >> 16;
     return k;
 }
 
