@@ -21,6 +21,7 @@
 //                    kernels, used when a hook replaces or observes G2P.
 //   k_rigid_update   rigid_impulses.slang:70-121.
 #include "common.h"
+#include "../../include/slosh_hip.h"
 
 namespace {
 
@@ -483,13 +484,52 @@ struct NodeColl {
 
 // collide(), collide.slang:87-154 (primitive shapes; trimesh colliders are
 // rejected at state creation).
-__device__ inline NodeColl collide(const Colliders &cs, float h, float3 point) {
+// World-space box around the region where collider i can report a contact
+// (the shape grown by `reach` along each axis, plus a margin far above the
+// rounding of collide()'s transforms). A point outside it is outside the shape
+// and farther than `reach` from its projection along some axis, so collide()
+// takes nothing from the collider there and skips it. Shapes whose projection
+// is the point itself never report a contact: empty box.
+__device__ inline void collider_box(const Colliders &cs, uint32_t i, float reach, float4 *lo, float4 *hi) {
+    const float4 a = cs.shapes[2 * i];
+    const uint32_t ty = __float_as_uint(a.w);
+    if (ty != 0u && ty != 1u) {
+        *lo = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+        *hi = make_float4(-1.0e30f, -1.0e30f, -1.0e30f, 0.0f);
+        return;
+    }
+    const float4 rot = cs.poses[2 * i], ts = cs.poses[2 * i + 1];
+    float3 ext;
+    if (ty == 1u) {
+        // Half extents of the rotated box along the world axes: |R| he.
+        const float3 cx = quat_rotate(rot, make_float3(1.0f, 0.0f, 0.0f));
+        const float3 cy = quat_rotate(rot, make_float3(0.0f, 1.0f, 0.0f));
+        const float3 cz = quat_rotate(rot, make_float3(0.0f, 0.0f, 1.0f));
+        ext = make_float3(fabsf(cx.x) * a.x + fabsf(cy.x) * a.y + fabsf(cz.x) * a.z,
+                          fabsf(cx.y) * a.x + fabsf(cy.y) * a.y + fabsf(cz.y) * a.z,
+                          fabsf(cx.z) * a.x + fabsf(cy.z) * a.y + fabsf(cz.z) * a.z);
+    } else {
+        ext = make_float3(a.x, a.x, a.x);
+    }
+    const float sc = fabsf(ts.w);
+    ext = make_float3(ext.x * sc, ext.y * sc, ext.z * sc);
+    const float size = fmaxf(fmaxf(ext.x, ext.y), ext.z) + fmaxf(fmaxf(fabsf(ts.x), fabsf(ts.y)), fabsf(ts.z));
+    const float m = reach * 1.01f + 1.0e-4f * size;
+    *lo = make_float4(ts.x - ext.x - m, ts.y - ext.y - m, ts.z - ext.z - m, 0.0f);
+    *hi = make_float4(ts.x + ext.x + m, ts.y + ext.y + m, ts.z + ext.z + m, 0.0f);
+}
+
+// `box`: collider_box of each collider (reach 1.5 h).
+__device__ inline NodeColl collide(const Colliders &cs, const float4 *box, float h, float3 point) {
     NodeColl c;
     c.normal_id = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
     c.point_valid = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
     float dist_cap = h * 1.5f;
     float best = 1.0e10f;
     for (uint32_t i = 0; i < cs.count; i++) {
+        const float4 lo = box[2 * i], hi = box[2 * i + 1];
+        if (point.x < lo.x || point.x > hi.x || point.y < lo.y || point.y > hi.y || point.z < lo.z || point.z > hi.z)
+            continue;
         float4 a = cs.shapes[2 * i];
         uint32_t ty = __float_as_uint(a.w);
         if (ty == 5u || ty == 6u) continue;
@@ -918,6 +958,9 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
                                                      const uint32_t *__restrict__ counters,
                                                      const uint32_t *__restrict__ blk_cnt, uint32_t mode,
                                                      uint32_t do_update) {
+    __shared__ float4 cbox[2 * SLOSH_MAX_COLLIDERS];
+    if (threadIdx.x < cs.count) collider_box(cs, threadIdx.x, 1.5f * P.h, &cbox[2 * threadIdx.x], &cbox[2 * threadIdx.x + 1]);
+    __syncthreads();
     const uint32_t B = counters[CTR_NUM_BLOCKS];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const int x = lane & 3, y = (lane >> 2) & 3, z = lane >> 4;
@@ -967,12 +1010,12 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
             if (P.coll_cache) {
                 uint32_t prev = nbr[(size_t)hid * NBR_STRIDE + NBR_PREV];
                 if (prev != SLOSH_NONE && coll_prev != nullptr) c = coll_prev[(size_t)prev * NODES_PER_BLOCK + lane];
-                else c = collide(cs, h, cell_pos);
+                else c = collide(cs, cbox, h, cell_pos);
                 coll_cur[(size_t)hid * NODES_PER_BLOCK + lane] = c;
             } else if (mass > 0.0f) {
                 // Static colliders: the cached record equals a fresh collide(); a
                 // zero-mass node has v = 0, which every boundary condition keeps.
-                c = collide(cs, h, cell_pos);
+                c = collide(cs, cbox, h, cell_pos);
             } else {
                 have = false;
             }
