@@ -287,6 +287,8 @@ static int alloc_particle_buffers(slosh_data *d, size_t pcap, bool sand) {
     TRY(dalloc(&d->pslot, pcap));
     TRY(dalloc(&d->prank, pcap));
     TRY(dalloc(&d->perm, pcap));
+    // Every entry a kernel can read is a particle index even before the first sort.
+    HIP_TRY(hipMemset(d->perm, 0, sizeof(uint32_t) * (pcap ? pcap : 1)));
     HIP_TRY(hipMemset(d->fdt, 0, sizeof(float) * 3 * (pcap ? pcap : 1)));
     return SLOSH_OK;
 }
