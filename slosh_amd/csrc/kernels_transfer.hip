@@ -1117,7 +1117,7 @@ __device__ inline void load_halo(float4 *__restrict__ sh, const float4 *__restri
 // F64: the SVD-based steps refined to FP64 accuracy. The bound keeps that
 // variant at three waves per SIMD (168 VGPRs).
 template <uint32_t MODELS, int NT, bool F64>
-__global__ __launch_bounds__(NT, F64 ? 3 : 1) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
+__global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
                                                     float *__restrict__ fdt, const uint32_t *__restrict__ perm,
                                                     const uint32_t *__restrict__ blk_start,
                                                     const uint32_t *__restrict__ blk_cnt,
@@ -1446,11 +1446,17 @@ hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, con
     st.model = s.model;
     st.dp0 = s.dp0;
     st.dp1 = s.dp1;
-    const bool nh_only = P.tag_mask == 2u;  // neo-Hookean only: no SVD code in the kernel
+    // The lightest instance that handles every material present: neo-Hookean
+    // only (no SVD code), fixed-corotated and neo-Hookean, every material.
+    const bool nh_only = (P.tag_mask & ~2u) == 0u;
+    const bool elastic_only = (P.tag_mask & ~3u) == 0u;
     const bool narrow = P.g2p_threads == 128u;
     auto kern = nh_only ? (narrow ? k_g2p_update<2u, 128, false> : k_g2p_update<2u, 256, false>)
-                        : (P.svd_f64 ? (narrow ? k_g2p_update<15u, 128, true> : k_g2p_update<15u, 256, true>)
-                                     : (narrow ? k_g2p_update<15u, 128, false> : k_g2p_update<15u, 256, false>));
+                : elastic_only
+                    ? (P.svd_f64 ? (narrow ? k_g2p_update<3u, 128, true> : k_g2p_update<3u, 256, true>)
+                                 : (narrow ? k_g2p_update<3u, 128, false> : k_g2p_update<3u, 256, false>))
+                    : (P.svd_f64 ? (narrow ? k_g2p_update<15u, 128, true> : k_g2p_update<15u, 256, true>)
+                                 : (narrow ? k_g2p_update<15u, 128, false> : k_g2p_update<15u, 256, false>));
     launch_kernel(ev, kern, dim3(block_grid(P, 1)), dim3(narrow ? 128 : 256), 0, stream, P, to_pbuf(in),
                   to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes,
                   t.counters, nt, b, t.hdr_key);
