@@ -968,20 +968,44 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
     for (uint32_t hid = blockIdx.x * 4 + wave; hid < B; hid += waves_per_grid) {
         float4 acc;
         if (mode == 0) {
-            // Tiles of blocks without particles were not written (all zero).
-            acc = blk_cnt[hid] ? tiles[(size_t)hid * TILE_NODES + x + 6 * y + 36 * z] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            // Tiles of blocks without particles were not written (they count as
+            // zero). Each tile value is loaded together with its block's particle
+            // count and then selected by it, so the tile loads wait only for the
+            // neighbour ids, not for the counts (every tile lies inside the
+            // allocation, written or not).
+            float4 t[8];
+            uint32_t c[8];
+            t[0] = tiles[(size_t)hid * TILE_NODES + x + 6 * y + 36 * z];
+            c[0] = blk_cnt[hid];
 #pragma unroll
             for (int d = 1; d < 8; d++) {
                 const int ox = ASSOC[d][0], oy = ASSOC[d][1], oz = ASSOC[d][2];
+                t[d] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                c[d] = 0u;
                 if ((ox == 0 || x <= 1) && (oy == 0 || y <= 1) && (oz == 0 || z <= 1)) {
-                    uint32_t src = nbr[(size_t)hid * NBR_STRIDE + NBR_MINUS + d - 1];
-                    if (src != SLOSH_NONE && blk_cnt[src] != 0u) {
-                        float4 t = tiles[(size_t)src * TILE_NODES + (x + 4 * ox) + 6 * (y + 4 * oy) + 36 * (z + 4 * oz)];
-                        acc.x += t.x;
-                        acc.y += t.y;
-                        acc.z += t.z;
-                        acc.w += t.w;
+                    const uint32_t src = nbr[(size_t)hid * NBR_STRIDE + NBR_MINUS + d - 1];
+                    if (src != SLOSH_NONE) {
+                        t[d] = tiles[(size_t)src * TILE_NODES + (x + 4 * ox) + 6 * (y + 4 * oy) + 36 * (z + 4 * oz)];
+                        c[d] = blk_cnt[src];
                     }
+                }
+            }
+            acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int d = 0; d < 8; d++) {
+                // Bitwise select: a stale tile (possibly NaN) never enters the sum.
+                const bool on = c[d] != 0u;
+                const float4 v = make_float4(__uint_as_float(on ? __float_as_uint(t[d].x) : 0u),
+                                             __uint_as_float(on ? __float_as_uint(t[d].y) : 0u),
+                                             __uint_as_float(on ? __float_as_uint(t[d].z) : 0u),
+                                             __uint_as_float(on ? __float_as_uint(t[d].w) : 0u));
+                if (d == 0) {
+                    acc = v;
+                } else {
+                    acc.x += v.x;
+                    acc.y += v.y;
+                    acc.z += v.z;
+                    acc.w += v.w;
                 }
             }
         } else {
