@@ -179,6 +179,20 @@ def hash_key(k):
     return lib().so_hash(k)
 
 
+def project_point_on_boundary(shape, pose, point):
+    """Shape::project_point_on_boundary (shape.slang:180-201): (handled, projection, inside)
+    for one collider shape record (layouts.SHAPE) at one pose (layouts.SIM3)."""
+    l = lib()
+    l.so_project_point_on_boundary.argtypes = [C.c_void_p] * 5
+    sh = np.array([shape], dtype=L.SHAPE)
+    po = np.array([pose], dtype=L.SIM3)
+    pt = np.ascontiguousarray(point, dtype=np.float32)
+    out = np.zeros(3, np.float32)
+    inside = C.c_int(0)
+    handled = l.so_project_point_on_boundary(_p(sh), _p(po), _p(pt), _p(out), C.byref(inside))
+    return bool(handled), out, bool(inside.value)
+
+
 def grid_update_node(bodies, mats, h, params, cell_pos, mvm):
     l = lib()
     l.so_grid_update_node.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_float, C.c_void_p, C.c_void_p,

@@ -522,6 +522,134 @@ static int ball_project_local_point_on_boundary(float radius, const float pt[3],
     return dist <= radius;
 }
 
+/* segment.slang:10-28: closest point of segment [a, b] to pt. */
+static void segment_project_local_point(const float a[3], const float b[3], const float pt[3], float out[3]) {
+    float ab[3], ap[3];
+    for (int i = 0; i < 3; i++) {
+        ab[i] = b[i] - a[i];
+        ap[i] = pt[i] - a[i];
+    }
+    const float ab_ap = dot3(ab, ap), sqnab = dot3(ab, ab);
+    if (ab_ap <= 0.0f) {
+        memcpy(out, a, sizeof(float) * 3);
+    } else if (ab_ap >= sqnab) {
+        memcpy(out, b, sizeof(float) * 3);
+    } else {
+        const float u = ab_ap / sqnab;
+        for (int i = 0; i < 3; i++) out[i] = a[i] + ab[i] * u;
+    }
+}
+
+/* capsule.slang:77-100: the first vector of the orthonormal basis built from unit v. */
+static void any_orthogonal_vector(const float v[3], float out[3]) {
+    const float sign = v[2] >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sign + v[2]);
+    const float b = v[0] * v[1] * a;
+    out[0] = 1.0f + sign * v[0] * v[0] * a;
+    out[1] = sign * b;
+    out[2] = -sign * v[0];
+}
+
+/* capsule.slang:41-58 (segment a..b, radius) */
+static int capsule_project_local_point_on_boundary(const float a[3], const float b[3], float radius, const float pt[3],
+                                                   float out[3]) {
+    float on_axis[3], d[3];
+    segment_project_local_point(a, b, pt, on_axis);
+    for (int i = 0; i < 3; i++) d[i] = pt[i] - on_axis[i];
+    const float dist = len3(d);
+    if (dist > 0.0f) {
+        const float k = radius / dist;
+        for (int i = 0; i < 3; i++) out[i] = on_axis[i] + d[i] * k;
+        return dist <= radius;
+    }
+    /* On the axis: any direction orthogonal to it. */
+    float ax[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const float len = len3(ax);
+    for (int i = 0; i < 3; i++) ax[i] /= len;
+    float dir[3];
+    any_orthogonal_vector(ax, dir);
+    for (int i = 0; i < 3; i++) out[i] = on_axis[i] + dir[i] * radius;
+    return 1;
+}
+
+/* Unit direction of (x, z) from the y axis, (1, 0) on the axis; returns |(x, z)|. */
+static float planar_dir(const float pt[3], float *dx, float *dz) {
+    const float pd = sqrtf(pt[0] * pt[0] + pt[2] * pt[2]);
+    if (pd > 0.0f) {
+        *dx = pt[0] / pd;
+        *dz = pt[2] / pd;
+    } else {
+        *dx = 1.0f;
+        *dz = 0.0f;
+    }
+    return pd;
+}
+
+/* cylinder.slang:68-128 (axis y, half height hh, radius r) */
+static int cylinder_project_local_point_on_boundary(float hh, float r, const float pt[3], float out[3]) {
+    float dx, dz;
+    const float pd = planar_dir(pt, &dx, &dz);
+    const float px = dx * r, pz = dz * r;
+    if (pt[1] >= -hh && pt[1] <= hh && pd <= r) {
+        const float to_top = hh - pt[1], to_bottom = pt[1] - (-hh), to_side = r - pd;
+        if (to_top < to_bottom && to_top < to_side) {
+            out[0] = pt[0]; out[1] = hh; out[2] = pt[2];
+        } else if (to_bottom < to_top && to_bottom < to_side) {
+            out[0] = pt[0]; out[1] = -hh; out[2] = pt[2];
+        } else {
+            out[0] = px; out[1] = pt[1]; out[2] = pz;
+        }
+        return 1;
+    }
+    if (pt[1] > hh || pt[1] < -hh) {
+        const float y = pt[1] > hh ? hh : -hh;
+        if (pd <= r) {
+            out[0] = pt[0]; out[1] = y; out[2] = pt[2];
+        } else {
+            out[0] = px; out[1] = y; out[2] = pz;
+        }
+    } else {
+        out[0] = px; out[1] = pt[1]; out[2] = pz;
+    }
+    return 0;
+}
+
+static void cross3(const float a[3], const float b[3], float out[3]) {
+    out[0] = a[1] * b[2] - a[2] * b[1];
+    out[1] = a[2] * b[0] - a[0] * b[2];
+    out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* cone.slang:61-120 (axis y, apex at +hh, base disc of radius r at -hh) */
+static int cone_project_local_point_on_boundary(float hh, float r, const float pt[3], float out[3]) {
+    float dx, dz;
+    const float pd = planar_dir(pt, &dx, &dz);
+    const float on_base[3] = {pt[0], -hh, pt[2]};
+    if (pt[1] < -hh && pd <= r) {
+        memcpy(out, on_base, sizeof on_base);
+        return 0;
+    }
+    const float rim[3] = {dx * r, -hh, dz * r};
+    const float apex[3] = {0.0f, hh, 0.0f};
+    const float side[3] = {rim[0] - apex[0], rim[1] - apex[1], rim[2] - apex[2]};
+    float proj[3];
+    segment_project_local_point(apex, rim, pt, proj);
+    const float apex_to_base[3] = {0.0f, -2.0f * hh, 0.0f};
+    const float ap[3] = {pt[0] - apex[0], pt[1] - apex[1], pt[2] - apex[2]};
+    float c1[3], c2[3];
+    cross3(side, ap, c1);
+    cross3(side, apex_to_base, c2);
+    if (pt[1] >= -hh && pt[1] <= hh && dot3(c1, c2) >= 0.0f) {
+        const float to_proj[3] = {proj[0] - pt[0], proj[1] - pt[1], proj[2] - pt[2]};
+        const float to_base[3] = {on_base[0] - pt[0], on_base[1] - pt[1], on_base[2] - pt[2]};
+        if (dot3(to_proj, to_proj) > dot3(to_base, to_base)) memcpy(out, on_base, sizeof on_base);
+        else memcpy(out, proj, sizeof proj);
+        return 1;
+    }
+    memcpy(out, proj, sizeof proj);
+    return 0;
+}
+
 /* shape.slang:180-201: returns handled (0 for polyline/trimesh/unknown). */
 static int shape_project_point_on_boundary(const so_shape *sh, const so_sim3 *pose, const float pt[3],
                                            float out[3], int *inside) {
@@ -533,6 +661,12 @@ static int shape_project_point_on_boundary(const so_shape *sh, const so_sim3 *po
         *inside = cuboid_project_local_point_on_boundary(sh->a, lp, lproj);
     } else if (ty == 0u) {
         *inside = ball_project_local_point_on_boundary(sh->a[0], lp, lproj);
+    } else if (ty == 2u) {
+        *inside = capsule_project_local_point_on_boundary(sh->a, sh->b, sh->b[3], lp, lproj);
+    } else if (ty == 3u) {
+        *inside = cone_project_local_point_on_boundary(sh->a[0], sh->a[1], lp, lproj);
+    } else if (ty == 4u) {
+        *inside = cylinder_project_local_point_on_boundary(sh->a[0], sh->a[1], lp, lproj);
     } else if (ty == 5u || ty == 6u) {
         return 0;
     } else {
@@ -542,6 +676,12 @@ static int shape_project_point_on_boundary(const so_shape *sh, const so_sim3 *po
     }
     sim_mul_pt(pose, lproj, out);
     return 1;
+}
+
+int so_project_point_on_boundary(const so_shape *sh, const so_sim3 *pose, const float pt[3], float out[3],
+                                 int *inside) {
+    *inside = 0;
+    return shape_project_point_on_boundary(sh, pose, pt, out, inside);
 }
 
 /* collide.slang:87-154 (non-trimesh branch) */

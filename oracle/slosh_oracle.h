@@ -110,6 +110,10 @@ void so_prefix_sum(uint32_t *v, size_t len);
 void so_grid_update_node(const so_body *bodies, const so_bc *mats, size_t nb, float h, const so_sim_params *params,
                          const float cell_pos[3], float mvm[4], float out_normal_dist[4]);
 uint32_t so_pack_key(int32_t x, int32_t y, int32_t z);
+/* Shape::project_point_on_boundary (shape.slang:180-201) for one collider shape
+ * and pose; returns 0 for the shapes collide() skips (polyline, trimesh). */
+int so_project_point_on_boundary(const so_shape *sh, const so_sim3 *pose, const float pt[3], float out[3],
+                                 int *inside);
 uint32_t so_hash(uint32_t packed_key);
 
 #ifdef __cplusplus

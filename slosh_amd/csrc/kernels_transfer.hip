@@ -470,6 +470,98 @@ __device__ inline bool ball_project_boundary(float radius, float3 pt, float3 *ou
     return dist <= radius;
 }
 
+// segment.slang:10-28: closest point of segment [a, b] to p.
+__device__ inline float3 segment_project(float3 a, float3 b, float3 p) {
+    const float3 ab = make_float3(b.x - a.x, b.y - a.y, b.z - a.z);
+    const float3 ap = make_float3(p.x - a.x, p.y - a.y, p.z - a.z);
+    const float ab_ap = ab.x * ap.x + ab.y * ap.y + ab.z * ap.z;
+    const float sqnab = ab.x * ab.x + ab.y * ab.y + ab.z * ab.z;
+    if (ab_ap <= 0.0f) return a;
+    if (ab_ap >= sqnab) return b;
+    const float u = ab_ap / sqnab;
+    return make_float3(a.x + ab.x * u, a.y + ab.y * u, a.z + ab.z * u);
+}
+
+// capsule.slang:41-58 and 77-100 (segment a..b, radius r).
+__device__ inline bool capsule_project_boundary(float3 a, float3 b, float r, float3 pt, float3 *out) {
+    const float3 on_axis = segment_project(a, b, pt);
+    const float3 d = make_float3(pt.x - on_axis.x, pt.y - on_axis.y, pt.z - on_axis.z);
+    const float dist = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    if (dist > 0.0f) {
+        const float k = r / dist;
+        *out = make_float3(on_axis.x + d.x * k, on_axis.y + d.y * k, on_axis.z + d.z * k);
+        return dist <= r;
+    }
+    // On the axis: the first vector of the orthonormal basis of the axis direction.
+    float3 ax = make_float3(b.x - a.x, b.y - a.y, b.z - a.z);
+    const float len = sqrtf(ax.x * ax.x + ax.y * ax.y + ax.z * ax.z);
+    ax = make_float3(ax.x / len, ax.y / len, ax.z / len);
+    const float sign = ax.z >= 0.0f ? 1.0f : -1.0f;
+    const float ka = -1.0f / (sign + ax.z);
+    const float kb = ax.x * ax.y * ka;
+    const float3 dir = make_float3(1.0f + sign * ax.x * ax.x * ka, sign * kb, -sign * ax.x);
+    *out = make_float3(on_axis.x + dir.x * r, on_axis.y + dir.y * r, on_axis.z + dir.z * r);
+    return true;
+}
+
+// |(x, z)| and the unit direction of (x, z) from the y axis ((1, 0) on the axis).
+__device__ inline float planar_dir(float3 pt, float *dx, float *dz) {
+    const float pd = sqrtf(pt.x * pt.x + pt.z * pt.z);
+    *dx = pd > 0.0f ? pt.x / pd : 1.0f;
+    *dz = pd > 0.0f ? pt.z / pd : 0.0f;
+    return pd;
+}
+
+// cylinder.slang:68-128 (axis y, half height hh, radius r).
+__device__ inline bool cylinder_project_boundary(float hh, float r, float3 pt, float3 *out) {
+    float dx, dz;
+    const float pd = planar_dir(pt, &dx, &dz);
+    const float px = dx * r, pz = dz * r;
+    if (pt.y >= -hh && pt.y <= hh && pd <= r) {
+        const float to_top = hh - pt.y, to_bottom = pt.y - (-hh), to_side = r - pd;
+        if (to_top < to_bottom && to_top < to_side) *out = make_float3(pt.x, hh, pt.z);
+        else if (to_bottom < to_top && to_bottom < to_side) *out = make_float3(pt.x, -hh, pt.z);
+        else *out = make_float3(px, pt.y, pz);
+        return true;
+    }
+    if (pt.y > hh || pt.y < -hh) {
+        const float y = pt.y > hh ? hh : -hh;
+        *out = pd <= r ? make_float3(pt.x, y, pt.z) : make_float3(px, y, pz);
+    } else {
+        *out = make_float3(px, pt.y, pz);
+    }
+    return false;
+}
+
+__device__ inline float3 cross3f(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// cone.slang:61-120 (axis y, apex at +hh, base disc of radius r at -hh).
+__device__ inline bool cone_project_boundary(float hh, float r, float3 pt, float3 *out) {
+    float dx, dz;
+    const float pd = planar_dir(pt, &dx, &dz);
+    const float3 on_base = make_float3(pt.x, -hh, pt.z);
+    if (pt.y < -hh && pd <= r) {
+        *out = on_base;
+        return false;
+    }
+    const float3 rim = make_float3(dx * r, -hh, dz * r);
+    const float3 apex = make_float3(0.0f, hh, 0.0f);
+    const float3 side = make_float3(rim.x - apex.x, rim.y - apex.y, rim.z - apex.z);
+    const float3 proj = segment_project(apex, rim, pt);
+    const float3 c1 = cross3f(side, make_float3(pt.x - apex.x, pt.y - apex.y, pt.z - apex.z));
+    const float3 c2 = cross3f(side, make_float3(0.0f, -2.0f * hh, 0.0f));
+    if (pt.y >= -hh && pt.y <= hh && c1.x * c2.x + c1.y * c2.y + c1.z * c2.z >= 0.0f) {
+        const float3 tp = make_float3(proj.x - pt.x, proj.y - pt.y, proj.z - pt.z);
+        const float3 tb = make_float3(on_base.x - pt.x, on_base.y - pt.y, on_base.z - pt.z);
+        *out = (tp.x * tp.x + tp.y * tp.y + tp.z * tp.z > tb.x * tb.x + tb.y * tb.y + tb.z * tb.z) ? on_base : proj;
+        return true;
+    }
+    *out = proj;
+    return false;
+}
+
 struct Colliders {
     const float4 *shapes; // 2 per body: a, b
     const float4 *poses;  // 2 per body: rotation, translation_scale
@@ -488,35 +580,44 @@ struct NodeColl {
 // (the shape grown by `reach` along each axis, plus a margin far above the
 // rounding of collide()'s transforms). A point outside it is outside the shape
 // and farther than `reach` from its projection along some axis, so collide()
-// takes nothing from the collider there and skips it. Shapes whose projection
-// is the point itself never report a contact: empty box.
+// takes nothing from the collider there and skips it. Polylines and trimeshes
+// (which collide() skips) and unknown shape types (whose projection is the
+// point itself: never a contact) get an empty box.
 __device__ inline void collider_box(const Colliders &cs, uint32_t i, float reach, float4 *lo, float4 *hi) {
     const float4 a = cs.shapes[2 * i];
     const uint32_t ty = __float_as_uint(a.w);
-    if (ty != 0u && ty != 1u) {
+    if (ty > 4u) {
         *lo = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
         *hi = make_float4(-1.0e30f, -1.0e30f, -1.0e30f, 0.0f);
         return;
     }
-    const float4 rot = cs.poses[2 * i], ts = cs.poses[2 * i + 1];
-    float3 ext;
+    // The shape's local bounding box (centre, half extents).
+    float3 c = make_float3(0.0f, 0.0f, 0.0f), e;
     if (ty == 1u) {
-        // Half extents of the rotated box along the world axes: |R| he.
-        const float3 cx = quat_rotate(rot, make_float3(1.0f, 0.0f, 0.0f));
-        const float3 cy = quat_rotate(rot, make_float3(0.0f, 1.0f, 0.0f));
-        const float3 cz = quat_rotate(rot, make_float3(0.0f, 0.0f, 1.0f));
-        ext = make_float3(fabsf(cx.x) * a.x + fabsf(cy.x) * a.y + fabsf(cz.x) * a.z,
-                          fabsf(cx.y) * a.x + fabsf(cy.y) * a.y + fabsf(cz.y) * a.z,
-                          fabsf(cx.z) * a.x + fabsf(cy.z) * a.y + fabsf(cz.z) * a.z);
+        e = make_float3(a.x, a.y, a.z);
+    } else if (ty == 0u) {
+        e = make_float3(a.x, a.x, a.x);
+    } else if (ty == 2u) {
+        const float4 b = cs.shapes[2 * i + 1];
+        c = make_float3(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z));
+        e = make_float3(0.5f * fabsf(b.x - a.x) + b.w, 0.5f * fabsf(b.y - a.y) + b.w, 0.5f * fabsf(b.z - a.z) + b.w);
     } else {
-        ext = make_float3(a.x, a.x, a.x);
+        e = make_float3(a.y, a.x, a.y);  // cone / cylinder: radius, half height (axis y), radius
     }
+    const float4 rot = cs.poses[2 * i], ts = cs.poses[2 * i + 1];
+    // World box of the rotated, scaled local box: centre transformed, half extents |R| e |scale|.
+    const float3 cw = sim_mul_pt(rot, ts, c);
+    const float3 cx = quat_rotate(rot, make_float3(1.0f, 0.0f, 0.0f));
+    const float3 cy = quat_rotate(rot, make_float3(0.0f, 1.0f, 0.0f));
+    const float3 cz = quat_rotate(rot, make_float3(0.0f, 0.0f, 1.0f));
     const float sc = fabsf(ts.w);
-    ext = make_float3(ext.x * sc, ext.y * sc, ext.z * sc);
-    const float size = fmaxf(fmaxf(ext.x, ext.y), ext.z) + fmaxf(fmaxf(fabsf(ts.x), fabsf(ts.y)), fabsf(ts.z));
+    const float3 ext = make_float3((fabsf(cx.x) * e.x + fabsf(cy.x) * e.y + fabsf(cz.x) * e.z) * sc,
+                                   (fabsf(cx.y) * e.x + fabsf(cy.y) * e.y + fabsf(cz.y) * e.z) * sc,
+                                   (fabsf(cx.z) * e.x + fabsf(cy.z) * e.y + fabsf(cz.z) * e.z) * sc);
+    const float size = fmaxf(fmaxf(ext.x, ext.y), ext.z) + fmaxf(fmaxf(fabsf(cw.x), fabsf(cw.y)), fabsf(cw.z));
     const float m = reach * 1.01f + 1.0e-4f * size;
-    *lo = make_float4(ts.x - ext.x - m, ts.y - ext.y - m, ts.z - ext.z - m, 0.0f);
-    *hi = make_float4(ts.x + ext.x + m, ts.y + ext.y + m, ts.z + ext.z + m, 0.0f);
+    *lo = make_float4(cw.x - ext.x - m, cw.y - ext.y - m, cw.z - ext.z - m, 0.0f);
+    *hi = make_float4(cw.x + ext.x + m, cw.y + ext.y + m, cw.z + ext.z + m, 0.0f);
 }
 
 // `box`: collider_box of each collider (reach 1.5 h).
@@ -536,11 +637,22 @@ __device__ inline NodeColl collide(const Colliders &cs, const float4 *box, float
         float4 rot = cs.poses[2 * i], ts = cs.poses[2 * i + 1];
         float3 proj;
         bool inside = false;
-        if (ty == 1u || ty == 0u) {
+        if (ty <= 4u) {
             float3 lp = sim_inv_mul_pt(rot, ts, point);
             float3 lproj;
-            if (ty == 1u) inside = cuboid_project_boundary(make_float3(a.x, a.y, a.z), lp, &lproj);
-            else inside = ball_project_boundary(a.x, lp, &lproj);
+            if (ty == 1u) {
+                inside = cuboid_project_boundary(make_float3(a.x, a.y, a.z), lp, &lproj);
+            } else if (ty == 0u) {
+                inside = ball_project_boundary(a.x, lp, &lproj);
+            } else if (ty == 2u) {
+                const float4 b = cs.shapes[2 * i + 1];
+                inside = capsule_project_boundary(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), b.w, lp,
+                                                  &lproj);
+            } else if (ty == 3u) {
+                inside = cone_project_boundary(a.x, a.y, lp, &lproj);
+            } else {
+                inside = cylinder_project_boundary(a.x, a.y, lp, &lproj);
+            }
             proj = sim_mul_pt(rot, ts, lproj);
         } else {
             proj = point; // shape.slang:200

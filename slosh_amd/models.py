@@ -9,7 +9,7 @@ same bytes the reference would upload:
   (src/models/drucker_prager.rs:35-55) and the default plastic state {1, 1, 0}
   (src/models/drucker_prager.rs:85-92)
 * ``GpuBoundaryCondition`` constants (src/solver/boundary_condition.rs:12-65)
-* ``GpuShape::cuboid`` / ``ball`` (src/rbd/shapes.rs:83-110)
+* ``GpuShape::cuboid`` / ``ball`` / ``capsule`` / ``cone`` / ``cylinder`` (src/rbd/shapes.rs:82-201)
 * ``ParticleDynamics::new``: V0 = (2r)^3, m = V0 * rho (src/solver/particle.rs:66-84)
 
 All arithmetic is float32, as the reference computes these in f32.
@@ -108,6 +108,34 @@ def ball(radius):
     sh = np.zeros(1, dtype=L.SHAPE)
     sh["a"][0, 0] = radius
     sh["a"][0, 3] = np.uint32(SHAPE_BALL).view(np.float32)
+    return sh[0]
+
+
+def capsule(a, b, radius):
+    """GpuShape::capsule (src/rbd/shapes.rs:120-132): segment a..b, radius in b.w."""
+    sh = np.zeros(1, dtype=L.SHAPE)
+    sh["a"][0, :3] = a
+    sh["a"][0, 3] = np.uint32(SHAPE_CAPSULE).view(np.float32)
+    sh["b"][0, :3] = b
+    sh["b"][0, 3] = radius
+    return sh[0]
+
+
+def cone(half_height, radius):
+    """GpuShape::cone (src/rbd/shapes.rs:181-187): axis y, apex at +half_height."""
+    sh = np.zeros(1, dtype=L.SHAPE)
+    sh["a"][0, 0] = half_height
+    sh["a"][0, 1] = radius
+    sh["a"][0, 3] = np.uint32(SHAPE_CONE).view(np.float32)
+    return sh[0]
+
+
+def cylinder(half_height, radius):
+    """GpuShape::cylinder (src/rbd/shapes.rs:195-201): axis y."""
+    sh = np.zeros(1, dtype=L.SHAPE)
+    sh["a"][0, 0] = half_height
+    sh["a"][0, 1] = radius
+    sh["a"][0, 3] = np.uint32(SHAPE_CYLINDER).view(np.float32)
     return sh[0]
 
 

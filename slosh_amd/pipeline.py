@@ -230,6 +230,12 @@ class MpmData:
         self.handle = h
         self.nbodies = len(bodies)
         self.batched = scene_ids is not None
+        self.cell_width = float(np.float32(cell_width))
+
+    @property
+    def params(self):
+        """The SimulationParams record last set from the host (creation or set_sim_params)."""
+        return self._params[0]
 
     @classmethod
     def from_scene(cls, pipeline, scene, **kw):

@@ -264,6 +264,39 @@ class Batch:
         return int(self.counts.sum())
 
 
+def shapes_test_scene(n_side=18, seed=4321, moving=True):
+    """Parity scene for the capsule, cone and cylinder colliders (collide.slang with
+    capsule.slang, cone.slang, cylinder.slang): a stirred neo-Hookean / fixed-corotated
+    block thrown down onto a rotated capsule, a cone and a tilted, scaled cylinder inside
+    the six walls; optionally the capsule moves (two-way body), which exercises the
+    collision cache with these shapes."""
+    dx = 1.0 / 64
+    spacing = dx / 2
+    side = n_side * spacing
+    origin = (0.5 - side / 2, 0.05, 0.5 - side / 2)  # the block encloses the three shapes
+    pos = lattice((n_side,) * 3, spacing, origin, 0.1 * dx, seed)
+    n = pos.shape[0]
+    u = uniform(seed + 1, n * 3).reshape(n, 3)
+    rel = pos.astype(np.float64) - np.array([0.5, 0.05 + side / 2, 0.5])
+    vel = np.cross(np.array([0.6, -0.4, 0.9]), rel) + np.array([0.0, -0.6, 0.0]) + (u - 0.5) * 0.05
+    tags = np.arange(n) % 2
+    models = np.stack([M.elastic_neo_hookean(3.0e4, 0.3), M.elastic(2.5e4, 0.3)])[tags]
+    parts = M.particle_arrays(pos, np.float32(dx / 4), 1000.0, models, vel.astype(np.float32))
+    bodies, mats = box_walls(M.SLIP)
+    bodies.append(M.body(M.capsule((-0.05, 0.0, -0.02), (0.06, 0.01, 0.03), 0.02), (0.47, 0.1, 0.5),
+                         M.quat_from_scaled_axis((0.4, -0.2, 0.7)),
+                         linvel=(0.1, 0.0, -0.05) if moving else (0, 0, 0), angvel=(0.0, 1.0, 0.5) if moving else (0, 0, 0),
+                         mass=0.4 if moving else None, inv_inertia_diag=(800.0, 800.0, 800.0) if moving else None))
+    mats.append(M.boundary_condition(M.SEPARATE, 0.2))
+    bodies.append(M.body(M.cone(0.04, 0.035), (0.55, 0.06, 0.46), M.quat_from_scaled_axis((0.0, 0.3, 0.0))))
+    mats.append(M.boundary_condition(M.STICK))
+    cyl = M.body(M.cylinder(0.03, 0.025), (0.52, 0.09, 0.56), M.quat_from_scaled_axis((1.0, 0.0, 0.4)))
+    cyl["pose"]["translation_scale"][3] = 1.25
+    bodies.append(cyl)
+    mats.append(M.boundary_condition(M.SLIP))
+    return Scene("shapes", parts, 1.0e-4, dx, 2048, bodies, mats, substeps=100)
+
+
 def mixed_test_scene(n_side=20, seed=12345, moving_body=True):
     """Parity scene that exercises every branch: all four material tags, nonzero
     velocity gradients and affine matrices, perturbed deformation gradients, a

@@ -168,6 +168,31 @@ def test_parity_sand_spheres_reduced(gpu_pipeline, steps):
         assert_sort_exact_on_gpu_state(sc, data, *gpu_pipeline)
 
 
+@pytest.mark.parametrize("steps", [1, 100])
+def test_parity_capsule_cone_cylinder(gpu_pipeline, steps):
+    # Capsule (moving, two-way: collision cache), cone and scaled cylinder colliders.
+    sc = scenes.shapes_test_scene()
+    data, orc = run_both(sc, steps, gpu_pipeline)
+    check_state(sc, data, orc, gpu_pipeline[1], steps)
+    check_blocks(sc, data, orc, gpu_pipeline, steps)
+
+
+def test_grid_nodes_match_oracle_shapes(gpu_pipeline):
+    # After one substep the grid nodes (momentum after P2G, velocity after the
+    # collision projection) equal the oracle's up to summation order.
+    sc = scenes.shapes_test_scene(moving=False)
+    data, orc = run_both(sc, 1, gpu_pipeline)
+    st = gpu_pipeline[1]
+    gb, ob = data.blocks(st), orc.blocks()
+    gn, on = data.nodes(st).reshape(-1, 64, 4), orc.nodes().reshape(-1, 64, 4)
+    oidx = {tuple(b["vid"]): i for i, b in enumerate(ob)}
+    perm = np.array([oidx[tuple(b["vid"])] for b in gb])
+    from tests.parity import rel_l2
+    e_v, e_m = rel_l2(gn[..., :3], on[perm][..., :3]), rel_l2(gn[..., 3], on[perm][..., 3])
+    print(f"\n[nodes, shapes] v={e_v:.3e} m={e_m:.3e}")
+    assert e_v <= 1e-5 and e_m <= 1e-5
+
+
 def test_grid_nodes_match_oracle(gpu_pipeline):
     sc = scenes.mixed_test_scene()
     data, orc = run_both(sc, 1, gpu_pipeline)
