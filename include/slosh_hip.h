@@ -205,6 +205,19 @@ int slosh_data_create(slosh_pipeline *pipeline, const slosh_sim_params *params,
                       uint32_t grid_capacity, slosh_data **out);
 void slosh_data_destroy(slosh_data *data);
 
+/* ---- Trimesh colliders: the body set's shared collision buffers
+ * (GpuBodySet::shapes_collision_vertices / _indices, src/rbd/dynamics/body.rs:84-90),
+ * as convert_trimesh_to_gpu (src/trimesh.rs:22-103) fills them: `vertices4` holds
+ * one Vect per entry as x, y, z, pad (the BVH node AABBs, the mesh vertices, the
+ * vertex and edge pseudo-normals), `indices` the BVH topology (entry, exit, shape
+ * per node) then the triangle vertex indices. Bodies whose shape is a trimesh
+ * (GpuShape::trimesh, src/rbd/shapes.rs:157-173) index into them. Copies both;
+ * every trimesh body's ranges, triangle indices and BVH links are validated (links
+ * must point forward, so the walk of collide() always ends): SLOSH_E_INVALID
+ * otherwise. A data with trimesh bodies and no buffers fails at the first step. */
+int slosh_data_set_collision_mesh(slosh_data *data, const float *vertices4, size_t num_vertices,
+                                  const uint32_t *indices, size_t num_indices);
+
 /* ---- One substep: MpmPipeline::launch_step (src/pipeline.rs:562-770).
  * Enqueues on `stream` and never synchronizes; `hooks` and `timestamps` may be NULL. */
 int slosh_launch_step(slosh_pipeline *pipeline, slosh_data *data, slosh_stream stream,

@@ -78,6 +78,12 @@ class Oracle:
                               scene.grid_capacity, svd_mode)
         if not self._s:
             raise MemoryError("oracle allocation failed")
+        mv, mi = getattr(scene, "collision_vertices", None), getattr(scene, "collision_indices", None)
+        if mv is not None and len(mv):
+            self._mesh = (np.ascontiguousarray(mv, dtype=np.float32), np.ascontiguousarray(mi, dtype=np.uint32))
+            l.so_set_collision_mesh.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t]
+            l.so_set_collision_mesh(self._s, _p(self._mesh[0]), self._mesh[0].shape[0], _p(self._mesh[1]),
+                                    self._mesh[1].size)
 
     def __del__(self):
         if getattr(self, "_s", None):
@@ -179,17 +185,30 @@ def hash_key(k):
     return lib().so_hash(k)
 
 
-def project_point_on_boundary(shape, pose, point):
-    """Shape::project_point_on_boundary (shape.slang:180-201): (handled, projection, inside)
-    for one collider shape record (layouts.SHAPE) at one pose (layouts.SIM3)."""
+class _Mesh(C.Structure):
+    _fields_ = [("vtx", C.c_void_p), ("idx", C.c_void_p), ("nv", C.c_size_t), ("ni", C.c_size_t)]
+
+
+def project_point_on_boundary(shape, pose, point, mesh=None, max_dist=1.0e30):
+    """Shape::project_point_on_boundary (shape.slang:180-201) and the trimesh branch of
+    collide (collide.slang:120-130): (handled, projection, inside) for one collider shape
+    record (layouts.SHAPE) at one pose (layouts.SIM3). `mesh` = (collision vertices (n, 4)
+    float32, collision indices u32) for trimesh shapes; `max_dist` bounds the trimesh query."""
     l = lib()
-    l.so_project_point_on_boundary.argtypes = [C.c_void_p] * 5
+    l.so_project_point_on_boundary.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                                               C.c_void_p, C.c_void_p]
     sh = np.array([shape], dtype=L.SHAPE)
     po = np.array([pose], dtype=L.SIM3)
     pt = np.ascontiguousarray(point, dtype=np.float32)
     out = np.zeros(3, np.float32)
     inside = C.c_int(0)
-    handled = l.so_project_point_on_boundary(_p(sh), _p(po), _p(pt), _p(out), C.byref(inside))
+    m = None
+    if mesh is not None:
+        mv = np.ascontiguousarray(mesh[0], dtype=np.float32)
+        mi = np.ascontiguousarray(mesh[1], dtype=np.uint32)
+        m = _Mesh(mv.ctypes.data, mi.ctypes.data, mv.shape[0], mi.size)
+    handled = l.so_project_point_on_boundary(_p(sh), _p(po), _p(pt), C.byref(m) if m is not None else None,
+                                             float(max_dist), _p(out), C.byref(inside))
     return bool(handled), out, bool(inside.value)
 
 

@@ -78,6 +78,7 @@ struct so_state {
     ncoll *coll, *prev_coll;
     int svd_mode;
     int overflow;
+    so_mesh mesh; /* trimesh collision buffers (shapes_collision_vertices / _indices) */
 };
 
 /* ------------------------------------------------------------------------- */
@@ -650,9 +651,143 @@ static int cone_project_local_point_on_boundary(float hh, float r, const float p
     return 0;
 }
 
-/* shape.slang:180-201: returns handled (0 for polyline/trimesh/unknown). */
+/* ---- triangle meshes (collision/trimesh.slang) ---- */
+
+enum { FEAT_VERTEX = 0, FEAT_EDGE = 1, FEAT_FACE = 2, FEAT_SOLID = 3 };
+
+/* triangle_project_local_point_and_get_location (trimesh.slang:304-377): closest
+ * point of triangle (a, b, c) to pt and its feature (vertex 0..2 = a, b, c; edge
+ * 0 = ab, 1 = bc, 2 = ca; face). */
+static void triangle_project(const float a[3], const float b[3], const float c[3], const float pt[3], float out[3],
+                             uint32_t *feat, uint32_t *id) {
+    float ab[3], ac[3], ap[3], bp[3], cp[3], bc[3];
+    for (int i = 0; i < 3; i++) {
+        ab[i] = b[i] - a[i];
+        ac[i] = c[i] - a[i];
+        ap[i] = pt[i] - a[i];
+    }
+    const float ab_ap = dot3(ab, ap), ac_ap = dot3(ac, ap);
+    if (ab_ap <= 0.0f && ac_ap <= 0.0f) {
+        memcpy(out, a, 12); *feat = FEAT_VERTEX; *id = 0;
+        return;
+    }
+    for (int i = 0; i < 3; i++) bp[i] = pt[i] - b[i];
+    const float ab_bp = dot3(ab, bp), ac_bp = dot3(ac, bp);
+    if (ab_bp >= 0.0f && ac_bp <= ab_bp) {
+        memcpy(out, b, 12); *feat = FEAT_VERTEX; *id = 1;
+        return;
+    }
+    for (int i = 0; i < 3; i++) cp[i] = pt[i] - c[i];
+    const float ab_cp = dot3(ab, cp), ac_cp = dot3(ac, cp);
+    if (ac_cp >= 0.0f && ab_cp <= ac_cp) {
+        memcpy(out, c, 12); *feat = FEAT_VERTEX; *id = 2;
+        return;
+    }
+    for (int i = 0; i < 3; i++) bc[i] = c[i] - b[i];
+    /* stable_check_edges_voronoi (trimesh.slang:257-302) */
+    float n[3], t[3];
+    cross3(ab, ac, n);
+    cross3(ab, ap, t);
+    const float vc = dot3(n, t);
+    if (vc < 0.0f && ab_ap >= 0.0f && ab_bp <= 0.0f) {
+        const float v = ab_ap / dot3(ab, ab);
+        for (int i = 0; i < 3; i++) out[i] = a[i] + ab[i] * v;
+        *feat = FEAT_EDGE; *id = 0;
+        return;
+    }
+    cross3(ac, cp, t);
+    const float vb = -dot3(n, t);
+    if (vb < 0.0f && ac_ap >= 0.0f && ac_cp <= 0.0f) {
+        const float w = ac_ap / dot3(ac, ac);
+        for (int i = 0; i < 3; i++) out[i] = a[i] + ac[i] * w;
+        *feat = FEAT_EDGE; *id = 2;
+        return;
+    }
+    cross3(bc, bp, t);
+    const float va = dot3(n, t);
+    if (va < 0.0f && ac_bp - ab_bp >= 0.0f && ab_cp - ac_cp >= 0.0f) {
+        const float w = dot3(bc, bp) / dot3(bc, bc);
+        for (int i = 0; i < 3; i++) out[i] = b[i] + bc[i] * w;
+        *feat = FEAT_EDGE; *id = 1;
+        return;
+    }
+    if (va + vb + vc != 0.0f) {
+        const float denom = 1.0f / (va + vb + vc);
+        const float v = vb * denom, w = vc * denom;
+        for (int i = 0; i < 3; i++) out[i] = a[i] + ab[i] * v + ac[i] * w;
+        *feat = FEAT_FACE; *id = 0;
+        return;
+    }
+    memcpy(out, pt, 12); /* degenerate triangle (trimesh.slang:372-376) */
+    *feat = FEAT_SOLID; *id = 0;
+}
+
+static const float *mesh_v(const so_mesh *m, uint32_t i) { return m->vtx + 4 * (size_t)i; }
+
+/* TriMesh::project_local_point (trimesh.slang:111-154): the closest triangle within
+ * max_dist by the stackless BVH walk, inside by the pseudo-normal of the feature
+ * the point projects on. Returns found. */
+static int trimesh_project_local_point(const so_mesh *m, const so_shape *sh, const float pt[3], float max_dist,
+                                       float out[3], int *inside) {
+    uint32_t vroot, iroot, nodes, ntri, nvert;
+    memcpy(&vroot, &sh->a[0], 4);
+    memcpy(&iroot, &sh->a[1], 4);
+    memcpy(&nodes, &sh->a[2], 4);
+    memcpy(&ntri, &sh->b[0], 4);
+    memcpy(&nvert, &sh->b[1], 4);
+    (void)ntri;
+    float best = max_dist, best_p[3] = {pt[0], pt[1], pt[2]};
+    uint32_t best_tri = 0, best_feat = FEAT_SOLID, best_id = 0;
+    int found = 0;
+    const uint32_t *tidx = m->idx + iroot + 3 * (size_t)nodes;
+    const uint32_t vbase = vroot + 2 * nodes;
+    uint32_t cur = 0;
+    while (cur < nodes) {
+        const uint32_t *node = m->idx + iroot + 3 * (size_t)cur;
+        if (node[0] == 0xffffffffu) {
+            const uint32_t tri = node[2];
+            float p[3];
+            uint32_t feat, id;
+            triangle_project(mesh_v(m, vbase + tidx[3 * tri]), mesh_v(m, vbase + tidx[3 * tri + 1]),
+                             mesh_v(m, vbase + tidx[3 * tri + 2]), pt, p, &feat, &id);
+            const float d[3] = {p[0] - pt[0], p[1] - pt[1], p[2] - pt[2]};
+            const float dist = len3(d);
+            if (dist <= best) {
+                best = dist;
+                memcpy(best_p, p, 12);
+                best_tri = tri; best_feat = feat; best_id = id;
+                found = 1;
+            }
+            cur = node[1];
+        } else {
+            const float *mn = mesh_v(m, vroot + 2 * cur), *mx = mesh_v(m, vroot + 2 * cur + 1);
+            float d[3];
+            for (int i = 0; i < 3; i++) d[i] = (pt[i] + (fmaxf(mn[i] - pt[i], 0.0f) - fmaxf(pt[i] - mx[i], 0.0f))) - pt[i];
+            cur = len3(d) < best ? node[0] : node[1];
+        }
+    }
+    /* TriMesh::pseudo_normal (trimesh.slang:83-109) */
+    float pn[3] = {0.0f, 0.0f, 0.0f};
+    const uint32_t *tv = tidx + 3 * (size_t)best_tri;
+    if (best_feat == FEAT_VERTEX) {
+        memcpy(pn, mesh_v(m, vbase + tv[best_id] + nvert), 12);
+    } else if (best_feat == FEAT_EDGE) {
+        memcpy(pn, mesh_v(m, vbase + 2 * nvert + 3 * best_tri + best_id), 12);
+    } else {
+        const float *a = mesh_v(m, vbase + tv[0]), *b = mesh_v(m, vbase + tv[1]), *c = mesh_v(m, vbase + tv[2]);
+        const float ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        cross3(ab, ac, pn);
+    }
+    const float dp[3] = {pt[0] - best_p[0], pt[1] - best_p[1], pt[2] - best_p[2]};
+    *inside = dot3(pn, dp) <= 0.0f;
+    memcpy(out, best_p, 12);
+    return found;
+}
+
+/* shape.slang:180-201 and the trimesh branch of collide.slang:120-130: returns
+ * handled (0 for polylines, and for trimeshes with no triangle within max_dist). */
 static int shape_project_point_on_boundary(const so_shape *sh, const so_sim3 *pose, const float pt[3],
-                                           float out[3], int *inside) {
+                                           const so_mesh *mesh, float max_dist, float out[3], int *inside) {
     uint32_t ty;
     memcpy(&ty, &sh->a[3], 4);
     float lp[3], lproj[3];
@@ -667,7 +802,9 @@ static int shape_project_point_on_boundary(const so_shape *sh, const so_sim3 *po
         *inside = cone_project_local_point_on_boundary(sh->a[0], sh->a[1], lp, lproj);
     } else if (ty == 4u) {
         *inside = cylinder_project_local_point_on_boundary(sh->a[0], sh->a[1], lp, lproj);
-    } else if (ty == 5u || ty == 6u) {
+    } else if (ty == 6u) {
+        if (!mesh || !mesh->vtx || !trimesh_project_local_point(mesh, sh, lp, max_dist, lproj, inside)) return 0;
+    } else if (ty == 5u) {
         return 0;
     } else {
         *inside = 0; /* shape.slang:200: unsupported shapes return the point itself */
@@ -678,10 +815,27 @@ static int shape_project_point_on_boundary(const so_shape *sh, const so_sim3 *po
     return 1;
 }
 
-int so_project_point_on_boundary(const so_shape *sh, const so_sim3 *pose, const float pt[3], float out[3],
-                                 int *inside) {
+int so_project_point_on_boundary(const so_shape *sh, const so_sim3 *pose, const float pt[3], const so_mesh *mesh,
+                                 float max_dist, float out[3], int *inside) {
     *inside = 0;
-    return shape_project_point_on_boundary(sh, pose, pt, out, inside);
+    return shape_project_point_on_boundary(sh, pose, pt, mesh, max_dist, out, inside);
+}
+
+void so_set_collision_mesh(so_state *s, const float *vtx4, size_t num_vertices, const uint32_t *idx,
+                           size_t num_indices) {
+    free((void *)s->mesh.vtx);
+    free((void *)s->mesh.idx);
+    s->mesh.vtx = NULL;
+    s->mesh.idx = NULL;
+    if (!num_vertices || !num_indices) return;
+    float *v = (float *)malloc(sizeof(float) * 4 * num_vertices);
+    uint32_t *i = (uint32_t *)malloc(sizeof(uint32_t) * num_indices);
+    memcpy(v, vtx4, sizeof(float) * 4 * num_vertices);
+    memcpy(i, idx, sizeof(uint32_t) * num_indices);
+    s->mesh.vtx = v;
+    s->mesh.idx = i;
+    s->mesh.nv = num_vertices;
+    s->mesh.ni = num_indices;
 }
 
 /* collide.slang:87-154 (non-trimesh branch) */
@@ -693,7 +847,9 @@ static ncoll collide(const so_state *s, const float point[3]) {
     for (size_t i = 0; i < s->nb && i < 16; i++) {
         float proj[3];
         int inside = 0;
-        if (!shape_project_point_on_boundary(&s->bodies[i].shape, &s->bodies[i].pose, point, proj, &inside)) continue;
+        if (!shape_project_point_on_boundary(&s->bodies[i].shape, &s->bodies[i].pose, point, &s->mesh, dist_cap, proj,
+                                             &inside))
+            continue;
         float dpt[3] = {proj[0] - point[0], proj[1] - point[1], proj[2] - point[2]};
         if (inside || (fabsf(dpt[0]) <= dist_cap && fabsf(dpt[1]) <= dist_cap && fabsf(dpt[2]) <= dist_cap)) {
             float sign = inside ? -1.0f : 1.0f;
@@ -1385,6 +1541,7 @@ void so_destroy(so_state *s) {
     free(s->pos); free(s->kin); free(s->F); free(s->props); free(s->models);
     free(s->bodies); free(s->mats); free(s->hmap); free(s->prev_hmap); free(s->hdr);
     free(s->scan); free(s->sorted); free(s->nodes); free(s->coll); free(s->prev_coll);
+    free((void *)s->mesh.vtx); free((void *)s->mesh.idx);
     free(s);
 }
 

@@ -50,6 +50,9 @@ typedef struct { uint32_t w[13]; } so_model; /* tag @0, payload @4 (52 B) */
 typedef struct { float gravity[3]; float dt; } so_sim_params;
 typedef struct { uint32_t ty; float friction; } so_bc;
 typedef struct { float a[4]; float b[4]; } so_shape;
+/* Trimesh collision buffers (convert_trimesh_to_gpu, src/trimesh.rs:22-103):
+ * vtx = x, y, z, pad per entry; idx = BVH topology then triangle indices. */
+typedef struct { const float *vtx; const uint32_t *idx; size_t nv, ni; } so_mesh;
 typedef struct { float rotation[4]; float translation_scale[4]; } so_sim3; /* quat xyzw; t.xyz, scale */
 typedef struct { float linear[3]; float pad0; float angular[3]; float pad1; } so_velocity;
 typedef struct { float inv_inertia[3][4]; float inv_mass[3]; float pad0; float com[3]; float pad1; } so_mprops;
@@ -112,8 +115,11 @@ void so_grid_update_node(const so_body *bodies, const so_bc *mats, size_t nb, fl
 uint32_t so_pack_key(int32_t x, int32_t y, int32_t z);
 /* Shape::project_point_on_boundary (shape.slang:180-201) for one collider shape
  * and pose; returns 0 for the shapes collide() skips (polyline, trimesh). */
-int so_project_point_on_boundary(const so_shape *sh, const so_sim3 *pose, const float pt[3], float out[3],
-                                 int *inside);
+int so_project_point_on_boundary(const so_shape *sh, const so_sim3 *pose, const float pt[3], const so_mesh *mesh,
+                                 float max_dist, float out[3], int *inside);
+/* Shared trimesh collision buffers of the body set (copied). */
+void so_set_collision_mesh(so_state *s, const float *vtx4, size_t num_vertices, const uint32_t *idx,
+                           size_t num_indices);
 uint32_t so_hash(uint32_t packed_key);
 
 #ifdef __cplusplus

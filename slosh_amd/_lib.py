@@ -12,7 +12,8 @@ ERRORS = {-1: "SLOSH_E_INVALID", -2: "SLOSH_E_HIP", -3: "SLOSH_E_NO_MEMORY", -4:
 # Every symbol include/slosh_hip.h declares.
 EXPORTS = [
     "slosh_pipeline_create", "slosh_pipeline_destroy", "slosh_kernels_default", "slosh_data_create",
-    "slosh_data_destroy", "slosh_launch_step", "slosh_launch_steps_graph", "slosh_set_svd_fp64", "slosh_set_sim_params",
+    "slosh_data_destroy", "slosh_data_set_collision_mesh", "slosh_launch_step", "slosh_launch_steps_graph",
+    "slosh_set_svd_fp64", "slosh_set_sim_params",
     "slosh_check_errors", "slosh_num_particles", "slosh_read_positions", "slosh_read_kinematics",
     "slosh_read_def_grad", "slosh_read_models", "slosh_read_properties", "slosh_read_scene_ids",
     "slosh_read_bodies", "slosh_num_blocks", "slosh_read_blocks", "slosh_read_nodes", "slosh_read_block_scenes",
@@ -91,6 +92,7 @@ def load():
         "slosh_launch_steps_graph": ([vp, vp, vp, u32], i32),
         "slosh_set_sim_params": ([vp, vp], i32),
         "slosh_set_svd_fp64": ([vp, i32], i32),
+        "slosh_data_set_collision_mesh": ([vp, vp, C.c_size_t, vp, C.c_size_t], i32),
         "slosh_check_errors": ([vp, vp], i32),
         "slosh_num_particles": ([vp], sz),
         "slosh_read_positions": ([vp, vp, vp], i32),

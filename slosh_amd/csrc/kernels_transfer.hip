@@ -417,8 +417,15 @@ __device__ inline void dp_project_f64(float *state, const float4 hp, float lambd
 
 // ---------------------------------------------------------------------------
 // Shapes, poses and collisions (shaders/slosh/rbd, shaders/slosh/collision).
+//
+// The contact geometry is evaluated without FMA contraction, with the operation
+// order of plain IEEE single precision. A contact normal is (projection - node) /
+// distance, so for a node near a surface its rounding error grows as 1 / distance:
+// contracting some products (the compiler's default) moves such normals by up to
+// 1e-4, where the uncontracted evaluation reproduces the CPU's bits.
 
 __device__ inline float3 quat_rotate(float4 q, float3 v) {
+#pragma clang fp contract(off)
     float3 t = make_float3(2.0f * (q.y * v.z - q.z * v.y), 2.0f * (q.z * v.x - q.x * v.z), 2.0f * (q.x * v.y - q.y * v.x));
     return make_float3(v.x + q.w * t.x + (q.y * t.z - q.z * t.y), v.y + q.w * t.y + (q.z * t.x - q.x * t.z),
                        v.z + q.w * t.z + (q.x * t.y - q.y * t.x));
@@ -426,10 +433,12 @@ __device__ inline float3 quat_rotate(float4 q, float3 v) {
 
 // Sim3 (quaternion, translation, uniform scale) point transforms.
 __device__ inline float3 sim_mul_pt(float4 rot, float4 ts, float3 p) {
+#pragma clang fp contract(off)
     float3 r = quat_rotate(rot, make_float3(p.x * ts.w, p.y * ts.w, p.z * ts.w));
     return make_float3(r.x + ts.x, r.y + ts.y, r.z + ts.z);
 }
 __device__ inline float3 sim_inv_mul_pt(float4 rot, float4 ts, float3 p) {
+#pragma clang fp contract(off)
     float3 r = quat_rotate(make_float4(-rot.x, -rot.y, -rot.z, rot.w), make_float3(p.x - ts.x, p.y - ts.y, p.z - ts.z));
     return make_float3(r.x / ts.w, r.y / ts.w, r.z / ts.w);
 }
@@ -438,6 +447,7 @@ __device__ inline float sgn(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0
 
 // cuboid.slang:34-63
 __device__ inline bool cuboid_project_boundary(float3 he, float3 pt, float3 *out) {
+#pragma clang fp contract(off)
     float3 op;
     op.x = pt.x + (fmaxf(-he.x - pt.x, 0.0f) - fmaxf(pt.x - he.x, 0.0f));
     op.y = pt.y + (fmaxf(-he.y - pt.y, 0.0f) - fmaxf(pt.y - he.y, 0.0f));
@@ -460,6 +470,7 @@ __device__ inline bool cuboid_project_boundary(float3 he, float3 pt, float3 *out
 
 // ball.slang:39-50
 __device__ inline bool ball_project_boundary(float radius, float3 pt, float3 *out) {
+#pragma clang fp contract(off)
     float dist = sqrtf(pt.x * pt.x + pt.y * pt.y + pt.z * pt.z);
     if (dist != 0.0f) {
         float k = radius / dist;
@@ -472,6 +483,7 @@ __device__ inline bool ball_project_boundary(float radius, float3 pt, float3 *ou
 
 // segment.slang:10-28: closest point of segment [a, b] to p.
 __device__ inline float3 segment_project(float3 a, float3 b, float3 p) {
+#pragma clang fp contract(off)
     const float3 ab = make_float3(b.x - a.x, b.y - a.y, b.z - a.z);
     const float3 ap = make_float3(p.x - a.x, p.y - a.y, p.z - a.z);
     const float ab_ap = ab.x * ap.x + ab.y * ap.y + ab.z * ap.z;
@@ -484,6 +496,7 @@ __device__ inline float3 segment_project(float3 a, float3 b, float3 p) {
 
 // capsule.slang:41-58 and 77-100 (segment a..b, radius r).
 __device__ inline bool capsule_project_boundary(float3 a, float3 b, float r, float3 pt, float3 *out) {
+#pragma clang fp contract(off)
     const float3 on_axis = segment_project(a, b, pt);
     const float3 d = make_float3(pt.x - on_axis.x, pt.y - on_axis.y, pt.z - on_axis.z);
     const float dist = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
@@ -506,6 +519,7 @@ __device__ inline bool capsule_project_boundary(float3 a, float3 b, float r, flo
 
 // |(x, z)| and the unit direction of (x, z) from the y axis ((1, 0) on the axis).
 __device__ inline float planar_dir(float3 pt, float *dx, float *dz) {
+#pragma clang fp contract(off)
     const float pd = sqrtf(pt.x * pt.x + pt.z * pt.z);
     *dx = pd > 0.0f ? pt.x / pd : 1.0f;
     *dz = pd > 0.0f ? pt.z / pd : 0.0f;
@@ -514,6 +528,7 @@ __device__ inline float planar_dir(float3 pt, float *dx, float *dz) {
 
 // cylinder.slang:68-128 (axis y, half height hh, radius r).
 __device__ inline bool cylinder_project_boundary(float hh, float r, float3 pt, float3 *out) {
+#pragma clang fp contract(off)
     float dx, dz;
     const float pd = planar_dir(pt, &dx, &dz);
     const float px = dx * r, pz = dz * r;
@@ -534,11 +549,13 @@ __device__ inline bool cylinder_project_boundary(float hh, float r, float3 pt, f
 }
 
 __device__ inline float3 cross3f(float3 a, float3 b) {
+#pragma clang fp contract(off)
     return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
 // cone.slang:61-120 (axis y, apex at +hh, base disc of radius r at -hh).
 __device__ inline bool cone_project_boundary(float hh, float r, float3 pt, float3 *out) {
+#pragma clang fp contract(off)
     float dx, dz;
     const float pd = planar_dir(pt, &dx, &dz);
     const float3 on_base = make_float3(pt.x, -hh, pt.z);
@@ -567,26 +584,150 @@ struct Colliders {
     const float4 *poses;  // 2 per body: rotation, translation_scale
     const uint2 *mats;    // (type, friction bits)
     uint32_t count;
+    const float4 *mesh_vtx;   // trimesh collision vertices: BVH AABBs, vertices, pseudo-normals
+    const uint32_t *mesh_idx; // trimesh collision indices: BVH topology, triangles
 };
+
+// triangle_project_local_point_and_get_location (trimesh.slang:304-377): closest
+// point of triangle (a, b, c) to p; feature 0 = vertex (id 0..2 = a, b, c),
+// 1 = edge (id 0 = ab, 1 = bc, 2 = ca), 2 = face, 3 = degenerate (the point itself).
+__device__ inline float3 triangle_project(float3 a, float3 b, float3 c, float3 p, uint32_t *feat, uint32_t *id) {
+#pragma clang fp contract(off)
+    const float3 ab = make_float3(b.x - a.x, b.y - a.y, b.z - a.z);
+    const float3 ac = make_float3(c.x - a.x, c.y - a.y, c.z - a.z);
+    const float3 ap = make_float3(p.x - a.x, p.y - a.y, p.z - a.z);
+    auto dot = [](float3 u, float3 v) { return u.x * v.x + u.y * v.y + u.z * v.z; };
+    const float ab_ap = dot(ab, ap), ac_ap = dot(ac, ap);
+    if (ab_ap <= 0.0f && ac_ap <= 0.0f) {
+        *feat = 0u; *id = 0u;
+        return a;
+    }
+    const float3 bp = make_float3(p.x - b.x, p.y - b.y, p.z - b.z);
+    const float ab_bp = dot(ab, bp), ac_bp = dot(ac, bp);
+    if (ab_bp >= 0.0f && ac_bp <= ab_bp) {
+        *feat = 0u; *id = 1u;
+        return b;
+    }
+    const float3 cp = make_float3(p.x - c.x, p.y - c.y, p.z - c.z);
+    const float ab_cp = dot(ab, cp), ac_cp = dot(ac, cp);
+    if (ac_cp >= 0.0f && ab_cp <= ac_cp) {
+        *feat = 0u; *id = 2u;
+        return c;
+    }
+    const float3 bc = make_float3(c.x - b.x, c.y - b.y, c.z - b.z);
+    // stable_check_edges_voronoi (trimesh.slang:257-302)
+    const float3 n = cross3f(ab, ac);
+    const float vc = dot(n, cross3f(ab, ap));
+    if (vc < 0.0f && ab_ap >= 0.0f && ab_bp <= 0.0f) {
+        const float v = ab_ap / dot(ab, ab);
+        *feat = 1u; *id = 0u;
+        return make_float3(a.x + ab.x * v, a.y + ab.y * v, a.z + ab.z * v);
+    }
+    const float vb = -dot(n, cross3f(ac, cp));
+    if (vb < 0.0f && ac_ap >= 0.0f && ac_cp <= 0.0f) {
+        const float w = ac_ap / dot(ac, ac);
+        *feat = 1u; *id = 2u;
+        return make_float3(a.x + ac.x * w, a.y + ac.y * w, a.z + ac.z * w);
+    }
+    const float va = dot(n, cross3f(bc, bp));
+    if (va < 0.0f && ac_bp - ab_bp >= 0.0f && ab_cp - ac_cp >= 0.0f) {
+        const float w = dot(bc, bp) / dot(bc, bc);
+        *feat = 1u; *id = 1u;
+        return make_float3(b.x + bc.x * w, b.y + bc.y * w, b.z + bc.z * w);
+    }
+    if (va + vb + vc != 0.0f) {
+        const float denom = 1.0f / (va + vb + vc);
+        const float v = vb * denom, w = vc * denom;
+        *feat = 2u; *id = 0u;
+        return make_float3(a.x + ab.x * v + ac.x * w, a.y + ab.y * v + ac.y * w, a.z + ab.z * v + ac.z * w);
+    }
+    *feat = 3u; *id = 0u;
+    return p;
+}
+
+// TriMesh::project_local_point (trimesh.slang:111-154): the closest triangle within
+// max_dist by the stackless BVH walk (links point forward, checked on the host, so
+// the walk ends after at most `nodes` steps), inside by the pseudo-normal of the
+// feature the point projects on. Returns whether a triangle was found.
+__device__ inline bool trimesh_project(const Colliders &cs, float4 a, float4 b, float3 pt, float max_dist,
+                                       float3 *out, bool *inside) {
+#pragma clang fp contract(off)
+    const uint32_t vroot = __float_as_uint(a.x), iroot = __float_as_uint(a.y), nodes = __float_as_uint(a.z);
+    const uint32_t nvert = __float_as_uint(b.y);
+    const uint32_t *topo = cs.mesh_idx + iroot;
+    const uint32_t *tri_idx = topo + 3u * nodes;
+    const float4 *verts = cs.mesh_vtx + vroot + 2u * nodes;
+    float best = max_dist;
+    float3 best_p = pt;
+    uint32_t best_tri = 0u, best_feat = 3u, best_id = 0u;
+    bool found = false;
+    uint32_t cur = 0u;
+    for (uint32_t steps = 0; cur < nodes && steps < nodes; steps++) {
+        const uint32_t entry = topo[3u * cur], exit_ = topo[3u * cur + 1u];
+        if (entry == 0xffffffffu) {
+            const uint32_t tri = topo[3u * cur + 2u];
+            const float4 va = verts[tri_idx[3u * tri]], vb = verts[tri_idx[3u * tri + 1u]],
+                         vc = verts[tri_idx[3u * tri + 2u]];
+            uint32_t feat, id;
+            const float3 p = triangle_project(make_float3(va.x, va.y, va.z), make_float3(vb.x, vb.y, vb.z),
+                                              make_float3(vc.x, vc.y, vc.z), pt, &feat, &id);
+            const float dx = p.x - pt.x, dy = p.y - pt.y, dz = p.z - pt.z;
+            const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+            if (dist <= best) {
+                best = dist;
+                best_p = p;
+                best_tri = tri; best_feat = feat; best_id = id;
+                found = true;
+            }
+            cur = exit_;
+        } else {
+            const float4 mn = cs.mesh_vtx[vroot + 2u * cur], mx = cs.mesh_vtx[vroot + 2u * cur + 1u];
+            const float qx = (pt.x + (fmaxf(mn.x - pt.x, 0.0f) - fmaxf(pt.x - mx.x, 0.0f))) - pt.x;
+            const float qy = (pt.y + (fmaxf(mn.y - pt.y, 0.0f) - fmaxf(pt.y - mx.y, 0.0f))) - pt.y;
+            const float qz = (pt.z + (fmaxf(mn.z - pt.z, 0.0f) - fmaxf(pt.z - mx.z, 0.0f))) - pt.z;
+            cur = sqrtf(qx * qx + qy * qy + qz * qz) < best ? entry : exit_;
+        }
+    }
+    if (!found) return false;
+    // TriMesh::pseudo_normal (trimesh.slang:83-109)
+    float3 pn;
+    const uint32_t *tv = tri_idx + 3u * best_tri;
+    if (best_feat == 0u) {
+        const float4 q = verts[tv[best_id] + nvert];
+        pn = make_float3(q.x, q.y, q.z);
+    } else if (best_feat == 1u) {
+        const float4 q = verts[2u * nvert + 3u * best_tri + best_id];
+        pn = make_float3(q.x, q.y, q.z);
+    } else {
+        const float4 va = verts[tv[0]], vb = verts[tv[1]], vc = verts[tv[2]];
+        pn = cross3f(make_float3(vb.x - va.x, vb.y - va.y, vb.z - va.z),
+                     make_float3(vc.x - va.x, vc.y - va.y, vc.z - va.z));
+    }
+    *inside = pn.x * (pt.x - best_p.x) + pn.y * (pt.y - best_p.y) + pn.z * (pt.z - best_p.z) <= 0.0f;
+    *out = best_p;
+    return true;
+}
 
 struct NodeColl {
     float4 normal_id; // normal.xyz, w = collider id (u32 bits)
     float4 point_valid; // point.xyz, w = valid (u32 bits)
 };
 
-// collide(), collide.slang:87-154 (primitive shapes; trimesh colliders are
-// rejected at state creation).
+// collide(), collide.slang:87-154 (primitive shapes and trimeshes; polylines
+// take no part, as in the reference).
 // World-space box around the region where collider i can report a contact
 // (the shape grown by `reach` along each axis, plus a margin far above the
 // rounding of collide()'s transforms). A point outside it is outside the shape
 // and farther than `reach` from its projection along some axis, so collide()
-// takes nothing from the collider there and skips it. Polylines and trimeshes
-// (which collide() skips) and unknown shape types (whose projection is the
-// point itself: never a contact) get an empty box.
+// takes nothing from the collider there and skips it. A trimesh's box is its
+// BVH root box. Polylines (which collide() skips) and unknown shape types (whose projection is the
+// point itself: never a contact) get an empty box. MESH: the launch carries the
+// trimesh buffers (otherwise no collider is a trimesh).
+template <bool MESH>
 __device__ inline void collider_box(const Colliders &cs, uint32_t i, float reach, float4 *lo, float4 *hi) {
     const float4 a = cs.shapes[2 * i];
     const uint32_t ty = __float_as_uint(a.w);
-    if (ty > 4u) {
+    if (ty > 4u && !(MESH && ty == 6u)) {
         *lo = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
         *hi = make_float4(-1.0e30f, -1.0e30f, -1.0e30f, 0.0f);
         return;
@@ -601,6 +742,12 @@ __device__ inline void collider_box(const Colliders &cs, uint32_t i, float reach
         const float4 b = cs.shapes[2 * i + 1];
         c = make_float3(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z));
         e = make_float3(0.5f * fabsf(b.x - a.x) + b.w, 0.5f * fabsf(b.y - a.y) + b.w, 0.5f * fabsf(b.z - a.z) + b.w);
+    } else if (MESH && ty == 6u) {
+        // The BVH root's box (the first node's AABB).
+        const uint32_t vroot = __float_as_uint(a.x);
+        const float4 mn = cs.mesh_vtx[vroot], mx = cs.mesh_vtx[vroot + 1u];
+        c = make_float3(0.5f * (mn.x + mx.x), 0.5f * (mn.y + mx.y), 0.5f * (mn.z + mx.z));
+        e = make_float3(0.5f * (mx.x - mn.x), 0.5f * (mx.y - mn.y), 0.5f * (mx.z - mn.z));
     } else {
         e = make_float3(a.y, a.x, a.y);  // cone / cylinder: radius, half height (axis y), radius
     }
@@ -621,7 +768,9 @@ __device__ inline void collider_box(const Colliders &cs, uint32_t i, float reach
 }
 
 // `box`: collider_box of each collider (reach 1.5 h).
+template <bool MESH>
 __device__ inline NodeColl collide(const Colliders &cs, const float4 *box, float h, float3 point) {
+#pragma clang fp contract(off)
     NodeColl c;
     c.normal_id = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
     c.point_valid = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
@@ -633,11 +782,18 @@ __device__ inline NodeColl collide(const Colliders &cs, const float4 *box, float
             continue;
         float4 a = cs.shapes[2 * i];
         uint32_t ty = __float_as_uint(a.w);
-        if (ty == 5u || ty == 6u) continue;
+        if (ty == 5u) continue;
         float4 rot = cs.poses[2 * i], ts = cs.poses[2 * i + 1];
         float3 proj;
         bool inside = false;
-        if (ty <= 4u) {
+        if (MESH && ty == 6u) {
+            // collide.slang:120-130: only a triangle within dist_cap counts.
+            float3 lproj;
+            if (!trimesh_project(cs, a, cs.shapes[2 * i + 1], sim_inv_mul_pt(rot, ts, point), dist_cap, &lproj,
+                                 &inside))
+                continue;
+            proj = sim_mul_pt(rot, ts, lproj);
+        } else if (ty <= 4u) {
             float3 lp = sim_inv_mul_pt(rot, ts, point);
             float3 lproj;
             if (ty == 1u) {
@@ -1063,6 +1219,7 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
 
 // mode 0: node momentum from tiles; mode 1: nodes already hold momentum (P2G hook).
 // do_update 0: only combine tiles (grid update disabled).
+template <bool MESH>
 __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 *__restrict__ tiles,
                                                      float4 *__restrict__ nodes, const uint32_t *__restrict__ nbr,
                                                      const uint32_t *__restrict__ hdr_key, Colliders cs,
@@ -1071,7 +1228,7 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
                                                      const uint32_t *__restrict__ blk_cnt, uint32_t mode,
                                                      uint32_t do_update) {
     __shared__ float4 cbox[2 * SLOSH_MAX_COLLIDERS];
-    if (threadIdx.x < cs.count) collider_box(cs, threadIdx.x, 1.5f * P.h, &cbox[2 * threadIdx.x], &cbox[2 * threadIdx.x + 1]);
+    if (threadIdx.x < cs.count) collider_box<MESH>(cs, threadIdx.x, 1.5f * P.h, &cbox[2 * threadIdx.x], &cbox[2 * threadIdx.x + 1]);
     __syncthreads();
     const uint32_t B = counters[CTR_NUM_BLOCKS];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -1146,12 +1303,12 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
             if (P.coll_cache) {
                 uint32_t prev = nbr[(size_t)hid * NBR_STRIDE + NBR_PREV];
                 if (prev != SLOSH_NONE && coll_prev != nullptr) c = coll_prev[(size_t)prev * NODES_PER_BLOCK + lane];
-                else c = collide(cs, cbox, h, cell_pos);
+                else c = collide<MESH>(cs, cbox, h, cell_pos);
                 coll_cur[(size_t)hid * NODES_PER_BLOCK + lane] = c;
             } else if (mass > 0.0f) {
                 // Static colliders: the cached record equals a fresh collide(); a
                 // zero-mass node has v = 0, which every boundary condition keeps.
-                c = collide(cs, cbox, h, cell_pos);
+                c = collide<MESH>(cs, cbox, h, cell_pos);
             } else {
                 have = false;
             }
@@ -1496,6 +1653,8 @@ struct ColliderBuffers {
     const float4 *shapes, *poses;
     const uint2 *mats;
     uint32_t count;
+    const float4 *mesh_vtx;    // trimesh collision vertices (x, y, z, pad)
+    const uint32_t *mesh_idx;  // trimesh collision indices
 };
 
 struct BodyBuffers {
@@ -1555,7 +1714,12 @@ hipError_t launch_grid_update(const StepParams &P, const TransferBuffers &t, con
     cs.poses = c.poses;
     cs.mats = c.mats;
     cs.count = c.count;
-    launch_kernel(ev, k_grid_update, dim3(block_grid(P, 4)), dim3(256), 0, stream, P,
+    cs.mesh_vtx = c.mesh_vtx;
+    cs.mesh_idx = c.mesh_idx;
+    // Trimesh projection (a BVH walk) is compiled into its own instance so that
+    // scenes without meshes keep the smaller kernel.
+    launch_kernel(ev, c.mesh_vtx ? k_grid_update<true> : k_grid_update<false>, dim3(block_grid(P, 4)), dim3(256), 0,
+                  stream, P,
                   (const float4 *)t.tiles, t.nodes, t.nbr, t.hdr_key, cs, (const NodeColl *)t.coll_prev,
                   (NodeColl *)t.coll_cur, t.counters, t.blk_cnt, mode, do_update);
     return hipGetLastError();

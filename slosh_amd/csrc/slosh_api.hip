@@ -58,6 +58,8 @@ struct ColliderBuffers {
     const float4 *shapes, *poses;
     const uint2 *mats;
     uint32_t count;
+    const float4 *mesh_vtx;    // trimesh collision vertices (x, y, z, pad)
+    const uint32_t *mesh_idx;  // trimesh collision indices
 };
 struct BodyBuffers {
     float4 *poses, *vels;
@@ -208,6 +210,12 @@ struct slosh_data {
     uint32_t nbodies = 0;
     std::vector<slosh_body_desc> bodies_host;
     float4 *shapes = nullptr, *poses = nullptr, *vels = nullptr, *mprops = nullptr, *local_com = nullptr;
+    // Trimesh collision buffers (slosh_data_set_collision_mesh); has_trimesh: some
+    // body is a trimesh, so the buffers are required.
+    float *mesh_vtx = nullptr;
+    uint32_t *mesh_idx = nullptr;
+    size_t mesh_nv = 0, mesh_ni = 0;
+    bool has_trimesh = false;
     uint2 *mats = nullptr;
     int *impulses = nullptr;
     // captured substep pairs, one per starting parity
@@ -620,6 +628,8 @@ static const PassEvents *pass_events(slosh_timestamps *ts, int pass, PassEvents 
 // ---- one substep --------------------------------------------------------------
 static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slosh_hooks *hooks, void *hs,
                     slosh_timestamps *ts) {
+    if (d->has_trimesh && !d->mesh_vtx)
+        return fail(SLOSH_E_INVALID, "trimesh colliders need slosh_data_set_collision_mesh before stepping");
     const slosh_kernels &K = pl->kernels;
     const bool any_hook = hooks && (hooks->after_particle_sort || hooks->run_p2g || hooks->run_g2p || hooks->after_p2g ||
                                     hooks->after_grid_update || hooks->after_g2p || hooks->after_particles_update);
@@ -701,6 +711,8 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     cb.poses = d->poses;
     cb.mats = d->mats;
     cb.count = d->nbodies;
+    cb.mesh_vtx = d->has_trimesh ? (const float4 *)d->mesh_vtx : nullptr;
+    cb.mesh_idx = d->has_trimesh ? d->mesh_idx : nullptr;
     if (K.grid_update || !p2g_by_hook) {
         const uint32_t mode = p2g_by_hook ? 1u : 0u;
         HIP_TRY(launch_grid_update(P, t, cb, mode, K.grid_update ? 1u : 0u, pass_events(ts, PASS_GRID_UPDATE, &pe),
@@ -807,6 +819,8 @@ static void free_data(slosh_data *d) {
     dfree(d->mats);
     dfree(d->impulses);
     dfree(d->dead);
+    dfree(d->mesh_vtx);
+    dfree(d->mesh_idx);
     dfree(d->dt_bound);
     if (d->capture_stream) (void)hipStreamDestroy(d->capture_stream);
     delete d;
@@ -868,7 +882,7 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     if (grid_capacity == 0 || grid_capacity > (1u << 26)) return fail(SLOSH_E_INVALID, "grid capacity out of range");
     for (size_t i = 0; i < num_bodies; i++) {
         uint32_t ty = f2u(bodies[i].shape.a[3]);
-        if (ty == 6u) return fail(SLOSH_E_UNSUPPORTED, "trimesh colliders are not implemented");
+        if (ty > 6u) return fail(SLOSH_E_UNSUPPORTED, "unknown collider shape type");
     }
     d->pipeline = pipeline;
     d->params = *params;
@@ -911,6 +925,7 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     TRY(dalloc(&d->tiles, cap * TILE_NODES));
     TRY(dalloc(&d->nodes, cap * NODES_PER_BLOCK));
     d->nbodies = (uint32_t)num_bodies;
+    for (size_t i = 0; i < num_bodies; i++) d->has_trimesh = d->has_trimesh || f2u(bodies[i].shape.a[3]) == 6u;
     d->bodies_host.assign(bodies, bodies + num_bodies);
     d->mats_host.assign(body_materials, body_materials + num_bodies);
     bool moving = false;
@@ -1070,6 +1085,54 @@ void slosh_graph_destroy(slosh_graph *g) {
     if (!g) return;
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     delete g;
+}
+
+int slosh_data_set_collision_mesh(slosh_data *d, const float *vertices4, size_t num_vertices,
+                                  const uint32_t *indices, size_t num_indices) {
+    if (!d) return fail(SLOSH_E_INVALID, "null data");
+    if ((num_vertices && !vertices4) || (num_indices && !indices)) return fail(SLOSH_E_INVALID, "null mesh buffers");
+    constexpr uint32_t LEAF = 0xffffffffu;
+    for (uint32_t b = 0; b < d->nbodies; b++) {
+        const slosh_shape &sh = d->bodies_host[b].shape;
+        if (f2u(sh.a[3]) != 6u) continue;
+        const uint64_t vroot = f2u(sh.a[0]), iroot = f2u(sh.a[1]), nodes = f2u(sh.a[2]);
+        const uint64_t ntri = f2u(sh.b[0]), nv = f2u(sh.b[1]);
+        if (nodes == 0 || ntri == 0 || nv < 3 || nodes > (1u << 28) || ntri > (1u << 28) || nv > (1u << 28))
+            return fail(SLOSH_E_INVALID, "trimesh body " + std::to_string(b) + ": empty or oversized mesh");
+        if (vroot + 2 * nodes + 2 * nv + 3 * ntri > num_vertices || iroot + 3 * nodes + 3 * ntri > num_indices)
+            return fail(SLOSH_E_INVALID, "trimesh body " + std::to_string(b) + ": ranges exceed the mesh buffers");
+        for (uint64_t k = 0; k < nodes; k++) {
+            const uint32_t *nd = indices + iroot + 3 * k;
+            const bool ok_entry = nd[0] == LEAF ? nd[2] < ntri : (nd[0] > k && nd[0] < nodes);
+            const bool ok_exit = nd[1] > k && nd[1] <= nodes;
+            if (!ok_entry || !ok_exit)
+                return fail(SLOSH_E_INVALID, "trimesh body " + std::to_string(b) + ": BVH node " + std::to_string(k) +
+                                                 " links backward or out of range");
+        }
+        // The grid update culls the collider by the first (root) node's box, so it
+        // must hold every triangle.
+        const float *root_mn = vertices4 + 4 * vroot, *root_mx = vertices4 + 4 * (vroot + 1);
+        for (uint64_t j = 0; j < 3 * ntri; j++) {
+            const uint32_t v = indices[iroot + 3 * nodes + j];
+            if (v >= nv)
+                return fail(SLOSH_E_INVALID, "trimesh body " + std::to_string(b) + ": triangle vertex out of range");
+            const float *p = vertices4 + 4 * (vroot + 2 * nodes + v);
+            for (int c = 0; c < 3; c++)
+                if (!(p[c] >= root_mn[c] && p[c] <= root_mx[c]))
+                    return fail(SLOSH_E_INVALID, "trimesh body " + std::to_string(b) +
+                                                     ": the root BVH box does not hold every triangle");
+        }
+    }
+    dfree(d->mesh_vtx);
+    dfree(d->mesh_idx);
+    TRY(dalloc(&d->mesh_vtx, 4 * std::max<size_t>(num_vertices, 1)));
+    TRY(dalloc(&d->mesh_idx, std::max<size_t>(num_indices, 1)));
+    if (num_vertices) HIP_TRY(hipMemcpy(d->mesh_vtx, vertices4, 16 * num_vertices, hipMemcpyHostToDevice));
+    if (num_indices) HIP_TRY(hipMemcpy(d->mesh_idx, indices, 4 * num_indices, hipMemcpyHostToDevice));
+    d->mesh_nv = num_vertices;
+    d->mesh_ni = num_indices;
+    invalidate_graphs(d);
+    return SLOSH_OK;
 }
 
 int slosh_set_svd_fp64(slosh_data *d, int enable) {
@@ -1728,6 +1791,17 @@ int slosh_autotune(slosh_pipeline *pl, slosh_data *d, slosh_stream stream, uint3
     TRY(slosh_data_create(pl, &d->params, &p, d->pcap, d->bodies_host.data(), d->nbodies, d->mats_host.data(), d->h,
                           d->cap, &c));
     c->svd_f64 = d->svd_f64;
+    if (d->mesh_vtx) {
+        std::vector<float> mv(4 * d->mesh_nv);
+        std::vector<uint32_t> mi(d->mesh_ni);
+        if (d->mesh_nv) HIP_TRY(hipMemcpy(mv.data(), d->mesh_vtx, 16 * d->mesh_nv, hipMemcpyDeviceToHost));
+        if (d->mesh_ni) HIP_TRY(hipMemcpy(mi.data(), d->mesh_idx, 4 * d->mesh_ni, hipMemcpyDeviceToHost));
+        const int rm = slosh_data_set_collision_mesh(c, mv.data(), d->mesh_nv, mi.data(), d->mesh_ni);
+        if (rm != SLOSH_OK) {
+            slosh_data_destroy(c);
+            return rm;
+        }
+    }
     slosh_tuning best = d->tuning;
     float best_ms = 0.0f;
     int rc = run_step(pl, c, st, nullptr, nullptr, nullptr);

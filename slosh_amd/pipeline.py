@@ -217,7 +217,7 @@ class MpmData:
     """Device-resident simulation state (MpmData::new)."""
 
     def __init__(self, pipeline, params, particles, bodies, materials, cell_width, grid_capacity,
-                 scene_ids=None, particle_capacity=0):
+                 scene_ids=None, particle_capacity=0, collision_mesh=None):
         st, keep = _particles_struct(particles, scene_ids)
         self._params = np.array([params], dtype=L.SIM_PARAMS)
         bodies = np.ascontiguousarray(np.asarray(bodies, dtype=L.BODY).reshape(-1))
@@ -231,6 +231,8 @@ class MpmData:
         self.nbodies = len(bodies)
         self.batched = scene_ids is not None
         self.cell_width = float(np.float32(cell_width))
+        if collision_mesh is not None:
+            self.set_collision_mesh(*collision_mesh)
 
     @property
     def params(self):
@@ -239,8 +241,18 @@ class MpmData:
 
     @classmethod
     def from_scene(cls, pipeline, scene, **kw):
+        if getattr(scene, "collision_vertices", None) is not None and "collision_mesh" not in kw:
+            kw["collision_mesh"] = (scene.collision_vertices, scene.collision_indices)
         return cls(pipeline, scene.params, scene.particles, scene.bodies, scene.materials, scene.cell_width,
                    scene.grid_capacity, **kw)
+
+    def set_collision_mesh(self, vertices4, indices):
+        """slosh_data_set_collision_mesh: the trimesh collision buffers ((n, 4) float32
+        x, y, z, pad; u32 indices) in convert_trimesh_to_gpu's layout."""
+        self._mesh = (np.ascontiguousarray(vertices4, dtype=np.float32).reshape(-1, 4),
+                      np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1))
+        _lib.check(_lib.load().slosh_data_set_collision_mesh(self.handle, _p(self._mesh[0]), self._mesh[0].shape[0],
+                                                             _p(self._mesh[1]), self._mesh[1].size))
 
     @classmethod
     def from_batch(cls, pipeline, batch, **kw):

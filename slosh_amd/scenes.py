@@ -297,6 +297,42 @@ def shapes_test_scene(n_side=18, seed=4321, moving=True):
     return Scene("shapes", parts, 1.0e-4, dx, 2048, bodies, mats, substeps=100)
 
 
+def trimesh_test_scene(n_side=18, seed=2468, moving=True):
+    """Parity scene for triangle-mesh colliders (collide.slang:120-130 with
+    collision/trimesh.slang): a stirred neo-Hookean block around a rotated box mesh
+    (moving two-way body when `moving`: collision cache) and a scaled icosphere mesh,
+    inside the six walls. The mesh buffers ride on the scene as collision_vertices /
+    collision_indices."""
+    from . import trimesh as TM
+    dx = 1.0 / 64
+    spacing = dx / 2
+    side = n_side * spacing
+    origin = (0.5 - side / 2, 0.05, 0.5 - side / 2)
+    pos = lattice((n_side,) * 3, spacing, origin, 0.1 * dx, seed)
+    n = pos.shape[0]
+    u = uniform(seed + 1, n * 3).reshape(n, 3)
+    rel = pos.astype(np.float64) - np.array([0.5, 0.05 + side / 2, 0.5])
+    vel = np.cross(np.array([-0.5, 0.7, 0.3]), rel) + np.array([0.0, -0.5, 0.0]) + (u - 0.5) * 0.05
+    parts = M.particle_arrays(pos, np.float32(dx / 4), 1000.0, M.elastic_neo_hookean(3.0e4, 0.3),
+                              vel.astype(np.float32))
+    bodies, mats = box_walls(M.SLIP)
+    buf = TM.CollisionBuffers()
+    box = TM.convert(*TM.box_mesh((0.035, 0.02, 0.03)), buf)
+    bodies.append(M.body(box, (0.47, 0.1, 0.49), M.quat_from_scaled_axis((0.5, 0.3, -0.4)),
+                         linvel=(0.08, 0.0, 0.05) if moving else (0, 0, 0), angvel=(0.5, 0.0, 1.0) if moving else (0, 0, 0),
+                         mass=0.4 if moving else None, inv_inertia_diag=(800.0, 800.0, 800.0) if moving else None))
+    mats.append(M.boundary_condition(M.SEPARATE, 0.2))
+    sph = TM.convert(*TM.icosphere(0.025, 1), buf)
+    body = M.body(sph, (0.54, 0.08, 0.55), M.quat_from_scaled_axis((0.0, 0.4, 0.2)))
+    body["pose"]["translation_scale"][3] = 1.2
+    bodies.append(body)
+    mats.append(M.boundary_condition(M.STICK))
+    sc = Scene("trimesh", parts, 1.0e-4, dx, 2048, bodies, mats, substeps=100)
+    sc.collision_vertices = buf.vertex_array()
+    sc.collision_indices = buf.index_array()
+    return sc
+
+
 def mixed_test_scene(n_side=20, seed=12345, moving_body=True):
     """Parity scene that exercises every branch: all four material tags, nonzero
     velocity gradients and affine matrices, perturbed deformation gradients, a

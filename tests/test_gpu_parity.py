@@ -193,6 +193,59 @@ def test_grid_nodes_match_oracle_shapes(gpu_pipeline):
     assert e_v <= 1e-5 and e_m <= 1e-5
 
 
+@pytest.mark.parametrize("steps", [1, 100])
+def test_parity_trimesh(gpu_pipeline, steps):
+    # Box mesh (moving, two-way: collision cache) and scaled icosphere mesh colliders.
+    sc = scenes.trimesh_test_scene()
+    data, orc = run_both(sc, steps, gpu_pipeline)
+    check_state(sc, data, orc, gpu_pipeline[1], steps)
+    check_blocks(sc, data, orc, gpu_pipeline, steps)
+
+
+def test_grid_nodes_match_oracle_trimesh(gpu_pipeline):
+    sc = scenes.trimesh_test_scene(moving=False)
+    data, orc = run_both(sc, 1, gpu_pipeline)
+    st = gpu_pipeline[1]
+    gb, ob = data.blocks(st), orc.blocks()
+    gn, on = data.nodes(st).reshape(-1, 64, 4), orc.nodes().reshape(-1, 64, 4)
+    oidx = {tuple(b["vid"]): i for i, b in enumerate(ob)}
+    perm = np.array([oidx[tuple(b["vid"])] for b in gb])
+    from tests.parity import rel_l2
+    e_v, e_m = rel_l2(gn[..., :3], on[perm][..., :3]), rel_l2(gn[..., 3], on[perm][..., 3])
+    print(f"\n[nodes, trimesh] v={e_v:.3e} m={e_m:.3e}")
+    assert e_v <= 1e-5 and e_m <= 1e-5
+
+
+def test_trimesh_buffers_are_validated(gpu_pipeline):
+    """slosh_data_set_collision_mesh rejects buffers the BVH walk could leave or
+    loop in, and stepping a trimesh scene without buffers fails."""
+    pl, st = gpu_pipeline
+    sc = scenes.trimesh_test_scene(moving=False)
+    vtx, idx = sc.collision_vertices, sc.collision_indices
+    data = P.MpmData.from_scene(pl, sc, collision_mesh=None)
+    with pytest.raises(Exception, match="collision_mesh"):
+        pl.launch_step(data, st)
+    box = sc.bodies[6]["shape"]
+    nodes = int(np.asarray(box["a"]).view(np.uint32)[2])
+    bad = idx.copy()
+    bad[3 * nodes] = 10_000  # triangle vertex out of range
+    with pytest.raises(Exception, match="triangle vertex"):
+        data.set_collision_mesh(vtx, bad)
+    bad = idx.copy()
+    bad[0] = 0  # root entry links to itself
+    with pytest.raises(Exception, match="links backward"):
+        data.set_collision_mesh(vtx, bad)
+    bad = vtx.copy()
+    bad[1, :3] = bad[0, :3]  # root box collapsed to a point
+    with pytest.raises(Exception, match="root BVH box"):
+        data.set_collision_mesh(bad, idx)
+    with pytest.raises(Exception, match="exceed"):
+        data.set_collision_mesh(vtx[:-1], idx)
+    data.set_collision_mesh(vtx, idx)
+    pl.launch_step(data, st)
+    data.check_errors(st)
+
+
 def test_grid_nodes_match_oracle(gpu_pipeline):
     sc = scenes.mixed_test_scene()
     data, orc = run_both(sc, 1, gpu_pipeline)
