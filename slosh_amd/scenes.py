@@ -333,6 +333,35 @@ def trimesh_test_scene(n_side=18, seed=2468, moving=True):
     return sc
 
 
+def heightfield_test_scene(n_side=18, seed=1357, sand=False):
+    """Parity scene for heightfield colliders (3D heightfields collide as the
+    trimesh of their samples, src/rbd/shapes.rs:252-268; cf. the reference's
+    heightfield3 example): a block falling onto a rotated, bumpy heightfield with a
+    stick boundary, inside the six walls. `sand`: Drucker-Prager sand instead of
+    neo-Hookean."""
+    from . import trimesh as TM
+    dx = 1.0 / 64
+    spacing = dx / 2
+    side = n_side * spacing
+    origin = (0.5 - side / 2, 0.075, 0.5 - side / 2)
+    pos = lattice((n_side,) * 3, spacing, origin, 0.1 * dx, seed)
+    n = pos.shape[0]
+    u = uniform(seed + 1, n * 3).reshape(n, 3)
+    vel = np.array([0.1, -1.0, 0.05]) + (u - 0.5) * 0.05
+    model = M.sand(1.0e5, 0.2) if sand else M.elastic_neo_hookean(3.0e4, 0.3)
+    parts = M.particle_arrays(pos, np.float32(dx / 4), 1500.0 if sand else 1000.0, model, vel.astype(np.float32))
+    bodies, mats = box_walls(M.SLIP)
+    i, j = np.meshgrid(np.arange(24), np.arange(20), indexing="ij")
+    buf = TM.CollisionBuffers()
+    hf = TM.convert(*TM.heightfield(np.sin(i / 3.0) * np.cos(j / 2.5), (0.4, 0.03, 0.45)), buf)
+    bodies.append(M.body(hf, (0.5, 0.07, 0.5), M.quat_from_scaled_axis((0.05, 0.3, -0.04))))
+    mats.append(M.boundary_condition(M.STICK))
+    sc = Scene("heightfield_sand" if sand else "heightfield", parts, 1.0e-4, dx, 2048, bodies, mats, substeps=100)
+    sc.collision_vertices = buf.vertex_array()
+    sc.collision_indices = buf.index_array()
+    return sc
+
+
 def mixed_test_scene(n_side=20, seed=12345, moving_body=True):
     """Parity scene that exercises every branch: all four material tags, nonzero
     velocity gradients and affine matrices, perturbed deformation gradients, a

@@ -175,6 +175,31 @@ def box_mesh(half_extents):
     return v, t
 
 
+def heightfield(heights, scale):
+    """A 3D heightfield as the triangle mesh the reference collides with.
+
+    GpuShape::from_parry (src/rbd/shapes.rs:252-268) turns a 3D heightfield into a
+    trimesh (parry's ``HeightField::to_trimesh``, then an oriented TriMesh). parry
+    is not vendored; this follows its documented layout: ``heights`` is an
+    (nrows, ncols) grid over the unit square centred at the origin, rows along z
+    and columns along x, scaled by ``scale``, so sample (i, j) sits at
+    x = (j / (ncols - 1) - 0.5) sx, y = heights[i, j] sy, z = (i / (nrows - 1) - 0.5) sz.
+    Each cell is split along the diagonal from sample (i + 1, j) to (i, j + 1), and
+    every triangle faces +y, so points below the surface are inside."""
+    hgt = np.asarray(heights, np.float64)
+    if hgt.ndim != 2 or min(hgt.shape) < 2:
+        raise ValueError("heights must be a 2D grid with at least 2 samples per axis")
+    nr, nc = hgt.shape
+    sx, sy, sz = (float(c) for c in scale)
+    i, j = np.meshgrid(np.arange(nr), np.arange(nc), indexing="ij")
+    v = np.stack([(j / (nc - 1) - 0.5) * sx, hgt * sy, (i / (nr - 1) - 0.5) * sz], axis=-1).reshape(-1, 3)
+    ci, cj = np.meshgrid(np.arange(nr - 1), np.arange(nc - 1), indexing="ij")
+    p00 = (ci * nc + cj).ravel()
+    p01, p10, p11 = p00 + 1, p00 + nc, p00 + nc + 1
+    t = np.concatenate([np.stack([p00, p10, p01], axis=1), np.stack([p10, p11, p01], axis=1)])
+    return v.astype(np.float32), t.astype(np.uint32)
+
+
 def icosphere(radius, subdivisions=1):
     """A closed, outward-oriented triangulated sphere."""
     p = (1.0 + 5 ** 0.5) / 2

@@ -216,6 +216,23 @@ def test_grid_nodes_match_oracle_trimesh(gpu_pipeline):
     assert e_v <= 1e-5 and e_m <= 1e-5
 
 
+@pytest.mark.parametrize("sand", [False, True])
+@pytest.mark.parametrize("steps", [1, 100])
+def test_parity_heightfield(gpu_pipeline, steps, sand):
+    # A block (neo-Hookean or sand) landing on a rotated bumpy heightfield (stick).
+    # Sand: F - I after a substep is ~1e-3 while the two SVD paths differ at 1e-7 of
+    # F, so (as for the sand bed) F is gated and F - I is not; the DP state is.
+    sc = scenes.heightfield_test_scene(sand=sand)
+    data, orc = run_both(sc, steps, gpu_pipeline)
+    check_state(sc, data, orc, gpu_pipeline[1], steps, diag=("dx",) if sand else ("dx", "F-I"))
+    check_blocks(sc, data, orc, gpu_pipeline, steps)
+    if sand:
+        from tests.parity import rel_l2
+        e = rel_l2(data.models(gpu_pipeline[1]).view(np.float32)[:, 1:4], orc.models().view(np.float32)[:, 1:4])
+        print(f"  DP state rel L2 = {e:.3e}")
+        assert e <= TOL[steps]
+
+
 def test_trimesh_buffers_are_validated(gpu_pipeline):
     """slosh_data_set_collision_mesh rejects buffers the BVH walk could leave or
     loop in, and stepping a trimesh scene without buffers fails."""
