@@ -349,6 +349,72 @@ __device__ inline void fc_stress_f64(float lambda, float mu, const float *f32, f
 // log(x) for x near 1 from the FP64 difference x - 1.
 __device__ inline double log_near1(double x) { return (double)log1pf((float)(x - 1.0)); }
 
+// Right singular vectors of f (the columns of v) to FP64 accuracy from FP32 ones
+// (the columns of v32): one Newton-Schulz step v = w (3 I - w^T w) / 2 makes them
+// orthonormal to FP64 round-off, then cyclic Jacobi sweeps on v^T f^T f v, whose
+// off-diagonal part starts at FP32 round-off, converge quadratically (at most 4
+// sweeps, also for repeated singular values). With FP32 vectors U diag(s') V^T
+// carries ~1e-7 absolute error, which is the size of a plastic F - I step.
+__device__ inline void refine_right_vectors(const double *f, const float *v32, double *v) {
+    double w[9], g[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) w[k] = v32[k];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) g[i * 3 + j] = w[0 * 3 + i] * w[0 * 3 + j] + w[1 * 3 + i] * w[1 * 3 + j] + w[2 * 3 + i] * w[2 * 3 + j];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            v[i * 3 + j] = 0.5 * (w[i * 3 + 0] * ((j == 0 ? 3.0 : 0.0) - g[0 * 3 + j]) +
+                                  w[i * 3 + 1] * ((j == 1 ? 3.0 : 0.0) - g[1 * 3 + j]) +
+                                  w[i * 3 + 2] * ((j == 2 ? 3.0 : 0.0) - g[2 * 3 + j]));
+    double b[9], a[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) b[i * 3 + j] = f[i * 3 + 0] * v[0 * 3 + j] + f[i * 3 + 1] * v[1 * 3 + j] + f[i * 3 + 2] * v[2 * 3 + j];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) a[i][j] = b[0 * 3 + i] * b[0 * 3 + j] + b[1 * 3 + i] * b[1 * 3 + j] + b[2 * 3 + i] * b[2 * 3 + j];
+#pragma unroll 1
+    for (int sweep = 0; sweep < 4; sweep++) {
+        const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        const double dg = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+        if (!(off > 1.0e-30 * dg)) break;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            const int p = e == 2 ? 1 : 0;
+            const int q = e == 0 ? 1 : 2;
+            const double apq = a[p][q];
+            if (apq == 0.0) continue;
+            const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+            const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double x = a[k][p], y = a[k][q];
+                a[k][p] = c * x - sn * y;
+                a[k][q] = sn * x + c * y;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double x = a[p][k], y = a[q][k];
+                a[p][k] = c * x - sn * y;
+                a[q][k] = sn * x + c * y;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double x = v[k * 3 + p], y = v[k * 3 + q];
+                v[k * 3 + p] = c * x - sn * y;
+                v[k * 3 + q] = sn * x + c * y;
+            }
+        }
+    }
+}
+
 // drucker_prager.slang:144-200 (Klar et al. 2016 return mapping in log-strain
 // space on the singular values), refined as described above.
 __device__ inline void dp_project_f64(float *state, const float4 hp, float lambda_f, float mu_f, float *f32) {
@@ -399,20 +465,64 @@ __device__ inline void dp_project_f64(float *state, const float4 hp, float lambd
     double dn = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
     double gamma = dn + (d * lambda + 2.0 * mu) / (2.0 * mu) * tr * alpha;
     if (gamma <= 0.0) return;
+    if (!(s32[2] > 0.0f)) {
+        // det F <= 0 (or NaN): log of the signed singular value is NaN, as in the
+        // shader; the particle is disabled by the update.
+#pragma unroll
+        for (int i = 0; i < 3; i++) ns[i] = 1.0 + (double)expm1f((float)(strain[i] - dev[i] * (gamma / dn)));
+        const double new_det = ns[0] * ns[1] * ns[2];
+        state[0] = (float)((double)state[0] * prev_det / new_det);
+        state[2] = (float)((double)state[2] + log_near1(prev_det) - log_near1(new_det));
+        state[1] = (float)((double)state[1] + gamma);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                f32[i * 3 + j] = (float)((double)u[i * 3 + 0] * ns[0] * (double)v[j * 3 + 0] +
+                                         (double)u[i * 3 + 1] * ns[1] * (double)v[j * 3 + 1] +
+                                         (double)u[i * 3 + 2] * ns[2] * (double)v[j * 3 + 2]);
+        return;
+    }
+    // Projection onto the cone: F' = U diag(s') V^T needs FP64-accurate singular
+    // vectors. Refine V, take the singular values |F v_i| from it and redo the
+    // return mapping with them (the decision above used the same values to second
+    // order), then F' = sum_i s'_i / |F v_i| (F v_i) v_i^T.
+    double fd[9], vd[9], bd[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) fd[k] = f32[k];
+    refine_right_vectors(fd, v, vd);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) bd[i * 3 + j] = fd[i * 3 + 0] * vd[0 * 3 + j] + fd[i * 3 + 1] * vd[1 * 3 + j] + fd[i * 3 + 2] * vd[2 * 3 + j];
+    double sr[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) sr[j] = sqrt(bd[0 * 3 + j] * bd[0 * 3 + j] + bd[1 * 3 + j] * bd[1 * 3 + j] + bd[2 * 3 + j] * bd[2 * 3 + j]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) strain[i] = log_near1(sr[i]) + (double)state[2] / d;
+    tr = strain[0] + strain[1] + strain[2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) dev[i] = strain[i] - tr / d;
+    dn = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+    gamma = dn + (d * lambda + 2.0 * mu) / (2.0 * mu) * tr * alpha;
+    if (gamma <= 0.0) return;
 #pragma unroll
     for (int i = 0; i < 3; i++) ns[i] = 1.0 + (double)expm1f((float)(strain[i] - dev[i] * (gamma / dn)));
     hard = gamma;
+    const double det_r = sr[0] * sr[1] * sr[2];
     const double new_det = ns[0] * ns[1] * ns[2];
-    state[0] = (float)((double)state[0] * prev_det / new_det);
-    state[2] = (float)((double)state[2] + log_near1(prev_det) - log_near1(new_det));
+    state[0] = (float)((double)state[0] * det_r / new_det);
+    state[2] = (float)((double)state[2] + log_near1(det_r) - log_near1(new_det));
     state[1] = (float)((double)state[1] + hard);
+    double rs[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) rs[j] = ns[j] / sr[j];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++)
-            f32[i * 3 + j] = (float)((double)u[i * 3 + 0] * ns[0] * (double)v[j * 3 + 0] +
-                                     (double)u[i * 3 + 1] * ns[1] * (double)v[j * 3 + 1] +
-                                     (double)u[i * 3 + 2] * ns[2] * (double)v[j * 3 + 2]);
+            f32[i * 3 + j] = (float)(bd[i * 3 + 0] * rs[0] * vd[j * 3 + 0] + bd[i * 3 + 1] * rs[1] * vd[j * 3 + 1] +
+                                     bd[i * 3 + 2] * rs[2] * vd[j * 3 + 2]);
 }
 
 // ---------------------------------------------------------------------------

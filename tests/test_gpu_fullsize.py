@@ -1,0 +1,133 @@
+"""Parity at the full sizes of the measured configurations.
+
+* The 1,048,576-particle dam break (2 particles per cell per axis, dx = 1/256,
+  weakly compressible fluid, separating walls): blocks bit-exact and x, v, affine,
+  F within relative L2 1e-5 after one substep; within 1e-3 after 100, with the
+  grid sort re-checked bit-exact on identical inputs.
+* The 4,194,304-particle sand bed + 64 neo-Hookean spheres + 8 two-way boxes on
+  the 512^3 grid: blocks bit-exact and the state within 1e-5 after one substep;
+  after 100, the sort re-checked bit-exact on identical inputs, no particle
+  disabled and mass conserved. Particle ids >= 4,194,240 are the ones the reference's capped
+  primary-block launch drops (sort.slang:60-66 with grid.rs:131-136); every
+  particle is inserted here and in the oracle, so the bit-exact block counts
+  cover them.
+* The 256-scene batch (11.4M particles): every scene against the oracle run on
+  that scene alone after one substep: blocks bit-exact, x, v, F within 1e-5 (these
+  scenes start at rest in free fall, where the APIC matrix is round-off, so it is
+  not gated; the stirred batches of test_gpu_batched gate it).
+"""
+import numpy as np
+import pytest
+
+from slosh_amd import pipeline as P
+from slosh_amd import scenes
+from oracle import oracle as O
+from tests.parity import assert_blocks_equal, assert_sort_exact_on_gpu_state, block_table, rel_l2, state_errors
+
+pytestmark = pytest.mark.gpu
+
+TOL = {1: 1.0e-5, 100: 1.0e-3}
+
+
+def gpu_state(data, st):
+    return data.positions(st), data.kinematics(st), data.def_grad(st)
+
+
+def oracle_state(orc):
+    return orc.positions(), orc.kinematics(), orc.def_grad()
+
+
+def gate(name, errs, tol, keys=("x", "v", "affine", "F")):
+    print(f"\n[{name}] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items() if k != "enabled_diff"), flush=True)
+    assert errs["enabled_diff"] == 0
+    for k in keys:
+        assert errs[k] <= tol, f"{name} {k}: relative L2 {errs[k]:.3e} > {tol}"
+
+
+@pytest.mark.parametrize("steps", [1, 100])
+def test_dam_break_full_size(gpu_pipeline, steps):
+    pl, st = gpu_pipeline
+    sc = scenes.dam_break()
+    assert sc.n == 1 << 20
+    data = P.MpmData.from_scene(pl, sc)
+    for _ in range(steps):
+        pl.launch_step(data, st)
+    data.check_errors(st)
+    orc = O.Oracle(sc)
+    orc.step(steps)
+    gate(f"dam break 1M after {steps}", state_errors(gpu_state(data, st), oracle_state(orc), sc.particles["pos4"]),
+         TOL[steps])
+    if steps == 1:
+        assert_blocks_equal(data.blocks(st), orc.blocks())
+    else:
+        g, r = block_table(data.blocks(st)), block_table(orc.blocks())
+        bad = (set(g) ^ set(r)) | {k for k in set(g) & set(r) if g[k] != r[k]}
+        print(f"  blocks that differ after {steps} substeps: {len(bad)} of {len(r)}")
+        assert len(bad) <= max(1, len(r) // 1000)
+        assert_sort_exact_on_gpu_state(sc, data, pl, st)
+
+
+def check_sand_spheres_step(sc, data, orc, st, name):
+    """One substep on both sides from the same state; sand F - I is not gated (the
+    two SVD paths differ at 1e-7 of F, F - I is ~1e-3 after a substep); the
+    Drucker-Prager state is."""
+    x0 = sc.particles["pos4"]
+    errs = state_errors(gpu_state(data, st), oracle_state(orc), x0)
+    gate(name, errs, TOL[1])
+    assert errs["dx"] <= 10 * TOL[1]
+    assert_blocks_equal(data.blocks(st), orc.blocks())
+    sand = orc.models()[:, 0] >= 2
+    e = rel_l2(data.models(st).view(np.float32)[sand, 1:4], orc.models().view(np.float32)[sand, 1:4])
+    print(f"  DP state rel L2 = {e:.3e}")
+    assert e <= TOL[1]
+
+
+def test_sand_spheres_full_size(gpu_pipeline):
+    pl, st = gpu_pipeline
+    sc = scenes.sand_spheres()
+    assert sc.n == 1 << 22
+    data = P.MpmData.from_scene(pl, sc)
+    pl.launch_step(data, st)
+    data.check_errors(st)
+    orc = O.Oracle(sc)
+    orc.step(1)
+    check_sand_spheres_step(sc, data, orc, st, "sand + spheres 4M after 1")
+    del orc
+    # After 100 substeps: the grid sort on identical inputs (the oracle restarted
+    # from the HIP state; blocks depend on positions only), no particle lost or
+    # disabled, mass conserved. A full restart comparison of the state is not
+    # made: the HIP path reuses its collision cache for blocks that existed in the
+    # previous substep (grid_update.slang:89-120), which a restarted oracle lacks
+    # while the boxes move.
+    for _ in range(99):
+        pl.launch_step(data, st)
+    data.check_errors(st)
+    kin = data.kinematics(st)
+    pos = data.positions(st)
+    assert np.all(np.isfinite(pos[:, :3]))
+    assert int(np.sum(kin["enabled"] != 0)) == sc.n
+    m0 = np.sum(sc.particles["kin"]["mass"].astype(np.float64))
+    assert abs(np.sum(kin["mass"].astype(np.float64)) - m0) <= 1e-6 * m0
+    assert_sort_exact_on_gpu_state(sc, data, pl, st)
+
+
+def test_batched_256_scenes_full_size(gpu_pipeline):
+    pl, st = gpu_pipeline
+    scs = [scenes.batched_scene(i) for i in range(256)]
+    b = scenes.Batch(scs)
+    assert b.n > 10_000_000
+    data = P.MpmData.from_batch(pl, b)
+    pl.launch_step(data, st)
+    data.check_errors(st)
+    blocks, bscene = data.blocks(st), data.block_scenes(st)
+    worst = {}
+    for sc, sid in zip(b.scenes, b.ids):
+        orc = O.Oracle(sc, svd_mode=1)
+        orc.step(1)
+        assert_blocks_equal(blocks[bscene == sid], orc.blocks())
+        errs = state_errors(data.read_scene(sid, st), oracle_state(orc), sc.particles["pos4"])
+        assert errs["enabled_diff"] == 0, f"scene {sid}"
+        for k in ("x", "v", "F"):
+            assert errs[k] <= TOL[1], f"scene {sid} (n={sc.n}) {k}: {errs[k]:.3e}"
+            worst[k] = max(worst.get(k, 0.0), errs[k])
+    print("\n[256-scene batch after 1] worst over scenes: " + " ".join(f"{k}={v:.3e}" for k, v in worst.items()))
