@@ -216,6 +216,11 @@ struct slosh_data {
     uint32_t *mesh_idx = nullptr;
     size_t mesh_nv = 0, mesh_ni = 0;
     bool has_trimesh = false;
+    // Pinned host arena that appends stage particles in (reused; stage_done marks
+    // the completion of the last upload from it).
+    uint8_t *stage_arena = nullptr;
+    size_t stage_bytes = 0;
+    hipEvent_t stage_done = nullptr;
     uint2 *mats = nullptr;
     int *impulses = nullptr;
     // captured substep pairs, one per starting parity
@@ -435,7 +440,35 @@ struct Staging {
     }
 };
 
-static int stage_particles(const slosh_particles *p, uint32_t id_base, Staging &s, bool *any_force, bool *any_sand) {
+// The same arrays carved from a pinned host arena (50 words per particle): host
+// to device copies from it run at the DMA rate and need not be waited for. The
+// arrays written as float4 come first (16 n bytes each), so each starts 16-byte
+// aligned for any count; the scalar planes follow.
+struct PinnedStaging {
+    static constexpr size_t WORDS = 50;
+    size_t n = 0;
+    float *pos, *vm, *C, *F, *vgd, *dps, *fdt, *prop, *model, *dp0, *dp1;
+    uint32_t *ids;
+    PinnedStaging(uint8_t *arena, size_t count) : n(count) {
+        float *w = reinterpret_cast<float *>(arena);
+        pos = w; w += 4 * count;
+        vm = w; w += 4 * count;
+        prop = w; w += 4 * count;
+        model = w; w += 4 * count;
+        dp0 = w; w += 4 * count;
+        dp1 = w; w += 4 * count;
+        C = w; w += 9 * count;
+        F = w; w += 9 * count;
+        vgd = w; w += count;
+        dps = w; w += 3 * count;
+        fdt = w; w += 3 * count;
+        ids = reinterpret_cast<uint32_t *>(w);
+    }
+    void resize(size_t) {}
+};
+
+template <typename S>
+static int stage_particles(const slosh_particles *p, uint32_t id_base, S &s, bool *any_force, bool *any_sand) {
     const size_t n = p->len;
     s.resize(n);
     for (size_t i = 0; i < n; i++) {
@@ -461,12 +494,12 @@ static int stage_particles(const slosh_particles *p, uint32_t id_base, Staging &
             s.fdt[(size_t)c * n + i] = k.force_dt[c];
             if (k.force_dt[c] != 0.0f) *any_force = true;
         }
+        for (int c = 0; c < 4; c++) s.dp0[4 * i + c] = s.dp1[4 * i + c] = 0.0f;
         static_records(pr, m, reinterpret_cast<float4 *>(&s.prop[4 * i]), reinterpret_cast<float4 *>(&s.model[4 * i]),
                        reinterpret_cast<float4 *>(&s.dp0[4 * i]), reinterpret_cast<float4 *>(&s.dp1[4 * i]));
-        if (model_is_sand(m)) {
-            *any_sand = true;
-            for (int c = 0; c < 3; c++) s.dps[(size_t)c * n + i] = u2f(m.words[1 + c]);
-        }
+        const bool sand = model_is_sand(m);
+        *any_sand = *any_sand || sand;
+        for (int c = 0; c < 3; c++) s.dps[(size_t)c * n + i] = sand ? u2f(m.words[1 + c]) : 0.0f;
     }
     return SLOSH_OK;
 }
@@ -488,31 +521,35 @@ static void static_records(const slosh_particle_properties &pr, const slosh_part
 }
 
 // Uploads staged particles into slots [slot0, slot0+n) of buffer `b` (ids starting at id0).
-static int upload_staging(slosh_data *d, const Staging &s, int b, size_t slot0, size_t id0, hipStream_t st) {
+// With `wait`, returns once the copies are done (the staging may then be freed);
+// otherwise the copies stay queued on `st` ahead of whatever follows.
+template <typename S>
+static int upload_staging(slosh_data *d, const S &s, int b, size_t slot0, size_t id0, hipStream_t st,
+                          bool wait = true) {
     const size_t n = s.n;
     if (n == 0) return SLOSH_OK;
     PBufs &p = d->pb[b];
     const size_t pc = d->pcap;
-    HIP_TRY(hipMemcpyAsync(p.pos + slot0, s.pos.data(), 16 * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(p.vm + slot0, s.vm.data(), 16 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p.pos + slot0, &s.pos[0], 16 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p.vm + slot0, &s.vm[0], 16 * n, hipMemcpyHostToDevice, st));
     for (int k = 0; k < 9; k++) {
-        HIP_TRY(hipMemcpyAsync(p.C + k * pc + slot0, s.C.data() + k * n, 4 * n, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(p.F + k * pc + slot0, s.F.data() + k * n, 4 * n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(p.C + k * pc + slot0, &s.C[k * n], 4 * n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(p.F + k * pc + slot0, &s.F[k * n], 4 * n, hipMemcpyHostToDevice, st));
     }
-    HIP_TRY(hipMemcpyAsync(p.vgd + slot0, s.vgd.data(), 4 * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(p.ids + slot0, s.ids.data(), 4 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p.vgd + slot0, &s.vgd[0], 4 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p.ids + slot0, &s.ids[0], 4 * n, hipMemcpyHostToDevice, st));
     if (p.dps)
         for (int k = 0; k < 3; k++)
-            HIP_TRY(hipMemcpyAsync(p.dps + k * pc + slot0, s.dps.data() + k * n, 4 * n, hipMemcpyHostToDevice, st));
+            HIP_TRY(hipMemcpyAsync(p.dps + k * pc + slot0, &s.dps[k * n], 4 * n, hipMemcpyHostToDevice, st));
     for (int k = 0; k < 3; k++)
-        HIP_TRY(hipMemcpyAsync(d->fdt + k * pc + slot0, s.fdt.data() + k * n, 4 * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d->s_prop + id0, s.prop.data(), 16 * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d->s_model + id0, s.model.data(), 16 * n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d->fdt + k * pc + slot0, &s.fdt[k * n], 4 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d->s_prop + id0, &s.prop[0], 16 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d->s_model + id0, &s.model[0], 16 * n, hipMemcpyHostToDevice, st));
     if (d->s_dp0) {
-        HIP_TRY(hipMemcpyAsync(d->s_dp0 + id0, s.dp0.data(), 16 * n, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(d->s_dp1 + id0, s.dp1.data(), 16 * n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d->s_dp0 + id0, &s.dp0[0], 16 * n, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d->s_dp1 + id0, &s.dp1[0], 16 * n, hipMemcpyHostToDevice, st));
     }
-    HIP_TRY(hipStreamSynchronize(st));
+    if (wait) HIP_TRY(hipStreamSynchronize(st));
     return SLOSH_OK;
 }
 
@@ -822,6 +859,11 @@ static void free_data(slosh_data *d) {
     dfree(d->mesh_vtx);
     dfree(d->mesh_idx);
     dfree(d->dt_bound);
+    if (d->stage_done) {
+        (void)hipEventSynchronize(d->stage_done);
+        (void)hipEventDestroy(d->stage_done);
+    }
+    if (d->stage_arena) (void)hipHostFree(d->stage_arena);
     if (d->capture_stream) (void)hipStreamDestroy(d->capture_stream);
     delete d;
 }
@@ -1513,17 +1555,41 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
     hipStream_t st = (hipStream_t)stream;
     if (particles->len == 0) return SLOSH_OK;
     bool any_force = false, any_sand = false;
-    Staging s;
-    TRY(stage_particles(particles, (uint32_t)d->n, s, &any_force, &any_sand));
-    // Host staging above overlaps substeps still running on `stream`; the graphs
-    // they replay are released only once they are done.
-    HIP_TRY(hipStreamSynchronize(st));
-    invalidate_graphs(d);
-    const bool need_sand = any_sand && !d->has_sand;
+    bool need_sand = false;
+    for (size_t i = 0; i < particles->len && !need_sand; i++) need_sand = model_is_sand(particles->models[i]);
+    need_sand = need_sand && !d->has_sand;
     if (d->n + particles->len <= d->pcap && !need_sand) {
-        TRY(upload_staging(d, s, d->cur_p, d->n, d->n, st));
+        // Into the free tail: stage in the pinned arena and do the host bookkeeping
+        // while substeps already queued on `stream` run, then queue the copies on
+        // `stream` (the next substeps follow them) without waiting for them. The
+        // wait for the queued substeps is for the graphs they replay, released
+        // only once they are done.
+        const size_t need = PinnedStaging::WORDS * 4 * particles->len;
+        if (d->stage_done) HIP_TRY(hipEventSynchronize(d->stage_done));
+        if (need > d->stage_bytes) {
+            if (d->stage_arena) HIP_TRY(hipHostFree(d->stage_arena));
+            d->stage_arena = nullptr;
+            d->stage_bytes = 0;
+            HIP_TRY(hipHostMalloc((void **)&d->stage_arena, need, hipHostMallocDefault));
+            d->stage_bytes = need;
+        }
+        if (!d->stage_done) HIP_TRY(hipEventCreateWithFlags(&d->stage_done, hipEventDisableTiming));
+        PinnedStaging s(d->stage_arena, particles->len);
+        TRY(stage_particles(particles, (uint32_t)d->n, s, &any_force, &any_sand));
+        host_append(d, particles);
+        note_records(d, particles->models, particles->properties, particles->len, 1);
+        HIP_TRY(hipStreamSynchronize(st));
+        invalidate_graphs(d);
+        TRY(upload_staging(d, s, d->cur_p, d->n, d->n, st, false));
+        HIP_TRY(hipEventRecord(d->stage_done, st));
         d->n += particles->len;
+        if (any_force) d->has_force = 1;
+        return SLOSH_OK;
     } else {
+        Staging s;
+        TRY(stage_particles(particles, (uint32_t)d->n, s, &any_force, &any_sand));
+        HIP_TRY(hipStreamSynchronize(st));
+        invalidate_graphs(d);
         // GpuTensor::append reallocates; we rebuild in original order with headroom
         // (rebuild sets d->n to the combined count).
         Staging all;
