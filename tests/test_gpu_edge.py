@@ -1,0 +1,113 @@
+"""Edge cases of the substep on the GPU, against the oracle.
+
+* A data with no particles steps (eagerly and through captured graphs) and reads
+  back empty arrays.
+* A single particle: blocks bit-exact and state within the one-substep gate, and
+  within the 100-substep gate while it falls onto the floor.
+* Particles handed in disabled: the reference's P2G does not test `enabled`
+  (p2g_scatter_style.slang:41-155 sums every sorted particle) and G2P gives them
+  zero velocity (g2p.slang:66-75), so their mass still reaches the grid.
+* Positions on rounding ties: x / h exactly k + 1/2, where the associated cell
+  round(x / h) - 1 uses round-half-to-even (WGSL round, grid.slang:346-355), and
+  positions exactly on block boundaries; the block sets and per-block counts must
+  be bit-exact.
+"""
+import numpy as np
+import pytest
+
+from slosh_amd import models as M
+from slosh_amd import pipeline as P
+from slosh_amd import scenes
+from oracle import oracle as O
+from tests.parity import assert_blocks_equal, state_errors
+
+pytestmark = pytest.mark.gpu
+
+TOL = {1: 1.0e-5, 100: 1.0e-3}
+
+
+def scene_from(pos, vel=None, model=None, name="edge", grid_capacity=256):
+    dx = 1.0 / 64
+    model = M.elastic_neo_hookean(1.0e5, 0.3) if model is None else model
+    parts = M.particle_arrays(np.asarray(pos, np.float32), np.float32(dx / 4), 1000.0, model,
+                              None if vel is None else np.asarray(vel, np.float32))
+    bodies, mats = scenes.box_walls(M.STICK)
+    return scenes.Scene(name, parts, 1.0e-4, dx, grid_capacity, bodies, mats)
+
+
+def run_both(sc, steps, gpu_pipeline):
+    pl, st = gpu_pipeline
+    data = P.MpmData.from_scene(pl, sc)
+    for _ in range(steps):
+        pl.launch_step(data, st)
+    data.check_errors(st)
+    orc = O.Oracle(sc)
+    orc.step(steps)
+    return data, orc
+
+
+def test_empty_data_steps(gpu_pipeline):
+    pl, st = gpu_pipeline
+    sc = scene_from(np.zeros((0, 3)))
+    data = P.MpmData.from_scene(pl, sc)
+    pl.launch_step(data, st)
+    pl.launch_steps_graph(data, 4, st)
+    data.check_errors(st)
+    assert data.n == 0
+    assert data.positions(st).shape[0] == 0
+    assert data.num_blocks(st) == 0
+
+
+@pytest.mark.parametrize("steps", [1, 100])
+def test_single_particle(gpu_pipeline, steps):
+    # Close above the floor wall (y = 0): it lands within 100 substeps.
+    sc = scene_from([[0.4031, 0.0142, 0.5517]], vel=[[0.3, -1.5, 0.2]])
+    data, orc = run_both(sc, steps, gpu_pipeline)
+    st = gpu_pipeline[1]
+    errs = state_errors((data.positions(st), data.kinematics(st), data.def_grad(st)),
+                        (orc.positions(), orc.kinematics(), orc.def_grad()), sc.particles["pos4"])
+    print(f"\n[one particle after {steps}] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()))
+    assert errs["enabled_diff"] == 0
+    for k in ("x", "v", "F"):
+        assert errs[k] <= TOL[steps], k
+    assert_blocks_equal(data.blocks(st), orc.blocks())
+
+
+def test_disabled_particles_still_reach_the_grid(gpu_pipeline):
+    sc = scenes.stir(scenes.elastic_cube(n_side=12), 5)
+    kin = sc.particles["kin"]
+    kin["enabled"][::7] = 0  # handed in disabled, mass kept
+    data, orc = run_both(sc, 1, gpu_pipeline)
+    st = gpu_pipeline[1]
+    gk, ok = data.kinematics(st), orc.kinematics()
+    off = kin["enabled"] == 0
+    assert np.all(gk["enabled"][off] == 0) and np.all(ok["enabled"][off] == 0)
+    assert np.all(np.asarray(gk["velocity"])[off] == 0.0)
+    errs = state_errors((data.positions(st), gk, data.def_grad(st)), (orc.positions(), ok, orc.def_grad()),
+                        sc.particles["pos4"])
+    print("\n[disabled particles] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()))
+    for k in ("x", "v", "affine", "F"):
+        assert errs[k] <= TOL[1], k
+    assert_blocks_equal(data.blocks(st), orc.blocks())
+    # Their mass is on the grid: the node mass sums agree with all particles' mass.
+    m_all = float(np.sum(kin["mass"].astype(np.float64)))
+    assert abs(float(np.sum(data.nodes(st).reshape(-1, 4)[:, 3].astype(np.float64))) - m_all) <= 1e-5 * m_all
+
+
+def test_rounding_ties_and_block_boundaries(gpu_pipeline):
+    h = 1.0 / 64
+    rng = np.random.default_rng(9)
+    k = rng.integers(16, 48, size=(3000, 3))
+    frac = rng.choice([0.5, 0.0, 0.25], size=(3000, 3), p=[0.5, 0.3, 0.2])
+    pos = (k + frac) * h  # exact in float32: h is a power of two
+    assert np.all(pos.astype(np.float32) / np.float32(h) == (k + frac))
+    sc = scene_from(pos, vel=np.zeros_like(pos), grid_capacity=1024)
+    data, orc = run_both(sc, 1, gpu_pipeline)
+    st = gpu_pipeline[1]
+    # After one substep the sort ran on the initial (tie) positions.
+    assert_blocks_equal(data.blocks(st), orc.blocks())
+    errs = state_errors((data.positions(st), data.kinematics(st), data.def_grad(st)),
+                        (orc.positions(), orc.kinematics(), orc.def_grad()), sc.particles["pos4"])
+    print("\n[ties] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()))
+    for key in ("x", "v", "F"):
+        assert errs[key] <= TOL[1], key
