@@ -285,7 +285,10 @@ int slosh_read_block_scenes(slosh_data *data, slosh_stream stream, uint32_t *out
  * Appending within the capacity given at creation uploads only the new particles;
  * beyond it the storage is reallocated. Removal runs on the device (order-preserving
  * compaction; the remaining particles keep their relative creation order and are
- * renumbered from 0). Both are enqueued on `stream` and invalidate captured graphs. */
+ * renumbered from 0). Both are enqueued on `stream` and invalidate captured graphs.
+ * An append within the capacity copies `particles` into a pinned host arena owned
+ * by `data` before it returns (the caller's arrays may be reused at once) and
+ * leaves the uploads queued on `stream`, ahead of the substeps launched next. */
 int slosh_append_particles(slosh_data *data, slosh_stream stream, const slosh_particles *particles);
 int slosh_remove_particles(slosh_data *data, slosh_stream stream, size_t start, size_t end);
 /* Removes every particle of `scene_id` / of the listed scenes (batched mode; no
