@@ -14,6 +14,12 @@
  *    affine-field reproduction (C == A exactly);
  *  - stress models: closed forms (F=I, F=R, symmetry, hand cases for DP);
  *  - 3x3 SVD: reconstruction / orthogonality against numpy's LAPACK SVD.
+ *  - collider projections: closed-form signed distances for ball, cuboid,
+ *    capsule, cone and cylinder (tests/test_oracle_shapes.py); trimesh and
+ *    heightfield projections by brute force over all triangles, a sloped-plane
+ *    closed form and the convex inside test (tests/test_oracle_trimesh.py,
+ *    tests/test_oracle_heightfield.py);
+ *  - timestep bound: closed forms per material (tests/test_oracle.py);
  *  - Sim3 / quaternion helpers of the third-party `stensor` crate (absent from
  *    the reference tree): textbook formulas, parity unpinned beyond identities.
  */
