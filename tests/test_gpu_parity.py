@@ -130,7 +130,9 @@ def test_parity_mixed(gpu_pipeline, steps):
 SHAPES = [{"p2g_chunk": 512, "g2p_threads": 256, "block_wgs": 4096},
           {"p2g_chunk": 256, "g2p_threads": 128, "block_wgs": 2048},
           {"p2g_chunk": 384, "g2p_threads": 256, "block_wgs": 16384},
-          {"p2g_chunk": 192, "g2p_threads": 128, "block_wgs": 8192}]
+          {"p2g_chunk": 192, "g2p_threads": 128, "block_wgs": 8192},
+          # Few persistent workgroups: every per-block kernel loops over many blocks.
+          {"p2g_chunk": 512, "g2p_threads": 256, "block_wgs": 64}]
 
 
 @pytest.mark.parametrize("shape", range(len(SHAPES)))
