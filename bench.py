@@ -284,7 +284,7 @@ def main():
     ap.add_argument("--no-batch-loop", dest="batch_loop", action="store_false",
                     help="skip the per-scene loop the batched leg is compared with")
     ap.add_argument("--serve-requests", type=int, default=32, help="requests per served run (0 = skip)")
-    ap.add_argument("--serve-rates", type=float, nargs="*", default=[10.0, 50.0, 200.0])
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[10.0, 20.0, 50.0, 100.0, 200.0])
     ap.add_argument("--hbm-budget", type=float, default=200e9)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shape instead of autotuning it before timing")
