@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1598,11 +1599,14 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
         const size_t need = PinnedStaging::WORDS * 4 * particles->len;
         if (d->stage_done) HIP_TRY(hipEventSynchronize(d->stage_done));
         if (need > d->stage_bytes) {
+            // Pinning is slow: grow geometrically so a stream of growing appends
+            // reallocates a few times only.
+            const size_t bytes = std::max(need, d->stage_bytes + d->stage_bytes / 2);
             if (d->stage_arena) HIP_TRY(hipHostFree(d->stage_arena));
             d->stage_arena = nullptr;
             d->stage_bytes = 0;
-            HIP_TRY(hipHostMalloc((void **)&d->stage_arena, need, hipHostMallocDefault));
-            d->stage_bytes = need;
+            HIP_TRY(hipHostMalloc((void **)&d->stage_arena, bytes, hipHostMallocDefault));
+            d->stage_bytes = bytes;
         }
         if (!d->stage_done) HIP_TRY(hipEventCreateWithFlags(&d->stage_done, hipEventDisableTiming));
         PinnedStaging s(d->stage_arena, particles->len);
@@ -1786,21 +1790,32 @@ int slosh_read_scene(slosh_data *d, slosh_stream stream, uint32_t scene_id, floa
     a.tile_sums = d->scan_tmp;
     a.total = d->scan_tmp + scan_tiles((uint32_t)d->pcap);
     HIP_TRY(launch_scene_gather(a, (uint32_t)d->n, (uint32_t)d->pcap, scene_id, (uint32_t)m, st));
-    std::vector<float> pos(4 * m), vm(4 * m), C(9 * m), F(9 * m), vgd(m), fdt(3 * m);
-    HIP_TRY(hipMemcpyAsync(pos.data(), stg.pos, 16 * m, hipMemcpyDeviceToHost, st));
-    if (kin) {
-        HIP_TRY(hipMemcpyAsync(vm.data(), stg.vm, 16 * m, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(C.data(), stg.C, 36 * m, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(vgd.data(), stg.vgd, 4 * m, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(fdt.data(), d->fdt2, 12 * m, hipMemcpyDeviceToHost, st));
+    // Positions go straight to the caller's buffer (their w, the flag bits, is
+    // cleared below); the other fields through temporaries allocated only when
+    // asked for and left uninitialised (the copies fill them).
+    std::unique_ptr<float[]> pos_tmp, vm, C, F, vgd, fdt;
+    float *pos = pos4;
+    if (!pos && kin) {
+        pos_tmp.reset(new float[4 * m]);
+        pos = pos_tmp.get();
     }
-    if (def_grad) HIP_TRY(hipMemcpyAsync(F.data(), stg.F, 36 * m, hipMemcpyDeviceToHost, st));
+    if (pos) HIP_TRY(hipMemcpyAsync(pos, stg.pos, 16 * m, hipMemcpyDeviceToHost, st));
+    if (kin) {
+        vm.reset(new float[4 * m]);
+        C.reset(new float[9 * m]);
+        vgd.reset(new float[m]);
+        fdt.reset(new float[3 * m]);
+        HIP_TRY(hipMemcpyAsync(vm.get(), stg.vm, 16 * m, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(C.get(), stg.C, 36 * m, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(vgd.get(), stg.vgd, 4 * m, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(fdt.get(), d->fdt2, 12 * m, hipMemcpyDeviceToHost, st));
+    }
+    if (def_grad) {
+        F.reset(new float[9 * m]);
+        HIP_TRY(hipMemcpyAsync(F.get(), stg.F, 36 * m, hipMemcpyDeviceToHost, st));
+    }
     HIP_TRY(hipStreamSynchronize(st));
     for (size_t j = 0; j < m; j++) {
-        if (pos4) {
-            for (int c = 0; c < 3; c++) pos4[4 * j + c] = pos[4 * j + c];
-            pos4[4 * j + 3] = 0.0f;
-        }
         if (kin) {
             slosh_kinematics &k = kin[j];
             std::memset(&k, 0, sizeof k);
@@ -1814,6 +1829,7 @@ int slosh_read_scene(slosh_data *d, slosh_stream stream, uint32_t scene_id, floa
             k.mass = vm[4 * j + 3];
             k.enabled = f2u(pos[4 * j + 3]) & 1u;
         }
+        if (pos4) pos4[4 * j + 3] = 0.0f;
         if (def_grad) {
             slosh_matrix &mm = def_grad[j];
             std::memset(&mm, 0, sizeof mm);
