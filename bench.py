@@ -20,6 +20,14 @@ workload (null for other workloads or when the summary is absent).
 
 cpu_baseline: the CPU oracle (a C restatement of the reference substep, OpenMP)
 timed on the host cores on the first S substeps of the same scene.
+
+The same line carries two more workloads of the metric: `batched` (256
+independent scenes, N log-uniform in [2e3, 2e5], neo-Hookean / fixed-corotated /
+Drucker-Prager, 200 substeps in one launch sequence, against stepping each scene
+alone) and `served` (32 scene requests, N log-uniform in [5e3, 2e6], 100-2000
+substeps each, Poisson arrivals at each rate of --serve-rates under a 200 GB HBM
+budget: p50 / p99 completion latency and aggregate particle-substeps/s, against
+serving the same requests one at a time).
 """
 import argparse
 import json
