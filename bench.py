@@ -60,16 +60,19 @@ def pass_bytes(n, n_blk, n_dp=0):
     }
 
 
-def measured_traffic(workload, pass_name):
-    """HBM bytes per launch of `pass_name`'s kernel from the committed PMC summary."""
+def measured_traffic(workload, pass_name, tuning):
+    """HBM bytes per launch of the kernel instance `pass_name` runs under `tuning`,
+    from the committed PMC summary (the dam break: neo-Hookean only, plain SVD path)."""
     if workload != "dam_break":
         return None
     path = os.path.join(ROOT, "profiles", "dam_break_1M_hbm_traffic.md")
     if not os.path.exists(path):
         return None
+    kernel = {"g2p": "k_g2p_update<2u,%d,false>" % tuning["g2p_threads"],
+              "p2g": "k_p2g<%d>" % tuning["p2g_chunk"], "grid_update": "k_grid_update<false>"}.get(pass_name)
     for ln in open(path):
         f = ln.split()
-        if len(f) == 4 and f[0] == "traffic" and f[1] == pass_name:
+        if len(f) == 4 and f[0] == "traffic" and f[1] == pass_name and f[2] == kernel:
             return float(f[3])
     return None
 
@@ -171,7 +174,7 @@ def run_plain(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.workload, dom),
+            "traffic": measured_traffic(args.workload, dom, tuning),
         },
         "passes": passes,
         "sum_of_pass_ms_per_step": timed_ms,
