@@ -51,6 +51,10 @@ struct StepParams {
     uint32_t tag_mask;    // material tags present (bit t = tag t; bit 4 = any other tag)
     uint32_t uniform;     // every particle has the same static record: use u_* below
     float4 u_prop, u_model, u_dp0, u_dp1;
+    // Every particle was created enabled with mass u_mass, and only the update has
+    // disabled particles since (it zeroes their mass): mass = enabled ? u_mass : 0.
+    uint32_t mass_uniform;
+    float u_mass;
     // Launch shape (autotuned): particles staged in LDS per P2G round, threads per
     // G2P/update workgroup, workgroups of the per-block kernels.
     uint32_t p2g_chunk, g2p_threads, block_wgs;

@@ -1556,7 +1556,9 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
             float aff[9], F[9];
             float vgd;
             g2p_particle(sh, P.h, P.inv_d, pos, enabled != 0u, &v, aff, &vgd);
-            float mass = in.vm[p].w;
+            // Uniform masses skip the 16-byte velocity / mass load (the velocity
+            // comes from the grid): only the update disables, and it zeroes the mass.
+            float mass = P.mass_uniform ? (enabled ? P.u_mass : 0.0f) : in.vm[p].w;
 #pragma unroll
             for (int k = 0; k < 9; k++) F[k] = in.F[(size_t)k * pcap + p];
             const uint32_t oid = in.ids[p];

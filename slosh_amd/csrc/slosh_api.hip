@@ -199,6 +199,10 @@ struct slosh_data {
     std::vector<HostRun> runs;
     uint64_t tag_count[5] = {0, 0, 0, 0, 0};  // particles per material tag (4: other)
     int uni_state = 0;                        // 0: no particles, 1: uniform (u_* valid), 2: mixed
+    // Particle masses: 0 no particles, 1 every particle created enabled with mass
+    // u_mass (bitwise) and not exposed to hooks or device views since, 2 otherwise.
+    int mass_state = 0;
+    float u_mass = 0.0f;
     // grid
     uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
     uint32_t *hdr_key = nullptr, *nbr = nullptr, *cell_cnt = nullptr, *cell_start = nullptr;
@@ -370,8 +374,25 @@ static void note_records(slosh_data *d, const slosh_particle_model *m, const slo
             }
         }
     }
-    if (d->n == 0 && sign < 0) d->uni_state = 0;
+    if (d->n == 0 && sign < 0) {
+        d->uni_state = 0;
+        d->mass_state = 0;
+    }
     d->uniform = d->uni_state == 1 ? 1u : 0u;
+}
+
+// Tracks whether every particle created so far was enabled with one mass.
+static void note_masses(slosh_data *d, const slosh_kinematics *k, size_t n) {
+    for (size_t i = 0; i < n && d->mass_state != 2; i++) {
+        if (!k[i].enabled) {
+            d->mass_state = 2;
+        } else if (d->mass_state == 0) {
+            d->u_mass = k[i].mass;
+            d->mass_state = 1;
+        } else if (f2u(k[i].mass) != f2u(d->u_mass)) {
+            d->mass_state = 2;
+        }
+    }
 }
 
 // Appends records to the host runs (split where the scene id changes).
@@ -650,6 +671,8 @@ static StepParams make_params(const slosh_data *d) {
     P.u_model = d->u_model;
     P.u_dp0 = d->u_dp0;
     P.u_dp1 = d->u_dp1;
+    P.mass_uniform = d->mass_state == 1 ? 1u : 0u;
+    P.u_mass = d->u_mass;
     return P;
 }
 
@@ -702,6 +725,8 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     const slosh_kernels &K = pl->kernels;
     const bool any_hook = hooks && (hooks->after_particle_sort || hooks->run_p2g || hooks->run_g2p || hooks->after_p2g ||
                                     hooks->after_grid_update || hooks->after_g2p || hooks->after_particles_update);
+    // Hooks may write particle masses through their device views.
+    if (any_hook) d->mass_state = 2;
     if (any_hook && d->nbodies > 0 && !d->coll_cache) {
         // Hooks may move bodies: from now on keep the reference's per-node cache.
         for (int i = 0; i < 2; i++)
@@ -976,6 +1001,7 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     TRY(alloc_particle_buffers(d, d->pcap, d->has_sand != 0));
     host_append(d, particles);
     note_records(d, particles->models, particles->properties, particles->len, 1);
+    note_masses(d, particles->kinematics, particles->len);
     HIP_TRY(hipStreamCreateWithFlags(&d->capture_stream, hipStreamNonBlocking));
     TRY(upload_staging(d, s, 0, 0, 0, d->capture_stream));
     const size_t cap = d->cap;
@@ -1613,6 +1639,7 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
         TRY(stage_particles(particles, (uint32_t)d->n, s, &any_force, &any_sand));
         host_append(d, particles);
         note_records(d, particles->models, particles->properties, particles->len, 1);
+        note_masses(d, particles->kinematics, particles->len);
         HIP_TRY(hipStreamSynchronize(st));
         invalidate_graphs(d);
         TRY(upload_staging(d, s, d->cur_p, d->n, d->n, st, false));
@@ -1636,6 +1663,7 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
     }
     host_append(d, particles);
     note_records(d, particles->models, particles->properties, particles->len, 1);
+    note_masses(d, particles->kinematics, particles->len);
     if (any_force) d->has_force = 1;
     return SLOSH_OK;
 }
@@ -2000,6 +2028,7 @@ int slosh_write_kinematics(slosh_data *d, slosh_stream stream, const slosh_kinem
 
 int slosh_device_view_get(slosh_data *d, slosh_device_view *out) {
     if (!d || !out) return fail(SLOSH_E_INVALID, "null argument");
+    d->mass_state = 2;  // the caller may write particle masses through the view
     PBufs &p = d->pb[d->cur_p];
     out->num_particles = d->n;
     out->particle_capacity = d->pcap;
