@@ -72,6 +72,16 @@ def check_blocks(scene, data, orc, gpu_pipeline, steps):
     assert len(bad) <= max(1, len(r) // 1000)
 
 
+def check_bodies(data, orc, st):
+    """Rigid-body poses and velocities (rigid_impulses.slang:70-121, body.slang:63-130)."""
+    gp, gv = data.bodies(st)
+    op, ov = orc.bodies()
+    assert np.allclose(gp["translation_scale"], op["translation_scale"], rtol=1e-5, atol=1e-6)
+    assert np.allclose(gp["rotation"], op["rotation"], rtol=1e-5, atol=1e-6)
+    assert np.allclose(gv["linear"], ov["linear"], rtol=1e-5, atol=1e-6)
+    assert np.allclose(gv["angular"], ov["angular"], rtol=1e-5, atol=1e-6)
+
+
 def test_blocks_bit_exact_elastic_cube(gpu_pipeline):
     sc = scenes.elastic_cube()
     data, orc = run_both(sc, 1, gpu_pipeline)
@@ -118,11 +128,7 @@ def test_parity_mixed(gpu_pipeline, steps):
     e = rel_l2(gm[sand, 1:4], om[sand, 1:4])
     print(f"  DP state rel L2 = {e:.3e}")
     assert e <= TOL[steps]
-    gp, gv = data.bodies(gpu_pipeline[1])
-    op, ov = orc.bodies()
-    assert np.allclose(gp["translation_scale"], op["translation_scale"], rtol=1e-5, atol=1e-6)
-    assert np.allclose(gp["rotation"], op["rotation"], rtol=1e-5, atol=1e-6)
-    assert np.allclose(gv["linear"], ov["linear"], rtol=1e-5, atol=1e-6)
+    check_bodies(data, orc, gpu_pipeline[1])
 
 
 # Every launch shape computes the same substep (P2G staging rounds of 192 / 256 /
@@ -161,6 +167,7 @@ def test_parity_sand_spheres_reduced(gpu_pipeline, steps):
     data, orc = run_both(sc, steps, gpu_pipeline)
     check_state(sc, data, orc, gpu_pipeline[1], steps, diag=("dx",))
     check_blocks(sc, data, orc, gpu_pipeline, steps)
+    check_bodies(data, orc, gpu_pipeline[1])
     sand = orc.models()[:, 0] >= 2
     from tests.parity import rel_l2
     e = rel_l2(data.models(gpu_pipeline[1]).view(np.float32)[sand, 1:4], orc.models().view(np.float32)[sand, 1:4])
@@ -176,6 +183,7 @@ def test_parity_capsule_cone_cylinder(gpu_pipeline, steps):
     sc = scenes.shapes_test_scene()
     data, orc = run_both(sc, steps, gpu_pipeline)
     check_state(sc, data, orc, gpu_pipeline[1], steps)
+    check_bodies(data, orc, gpu_pipeline[1])
     check_blocks(sc, data, orc, gpu_pipeline, steps)
 
 
@@ -201,6 +209,7 @@ def test_parity_trimesh(gpu_pipeline, steps):
     sc = scenes.trimesh_test_scene()
     data, orc = run_both(sc, steps, gpu_pipeline)
     check_state(sc, data, orc, gpu_pipeline[1], steps)
+    check_bodies(data, orc, gpu_pipeline[1])
     check_blocks(sc, data, orc, gpu_pipeline, steps)
 
 
