@@ -203,6 +203,7 @@ struct slosh_data {
     // u_mass (bitwise) and not exposed to hooks or device views since, 2 otherwise.
     int mass_state = 0;
     float u_mass = 0.0f;
+    bool masses_exposed = false;  // a hook ran or a device view was handed out
     // grid
     uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
     uint32_t *hdr_key = nullptr, *nbr = nullptr, *cell_cnt = nullptr, *cell_start = nullptr;
@@ -376,16 +377,17 @@ static void note_records(slosh_data *d, const slosh_particle_model *m, const slo
     }
     if (d->n == 0 && sign < 0) {
         d->uni_state = 0;
-        d->mass_state = 0;
+        if (!d->masses_exposed) d->mass_state = 0;
     }
     d->uniform = d->uni_state == 1 ? 1u : 0u;
 }
 
-// Tracks whether every particle created so far was enabled with one mass.
+// Tracks whether every particle given so far has mass = enabled ? u_mass : +0
+// (bitwise), the form the fused update keeps (it zeroes the mass it disables).
 static void note_masses(slosh_data *d, const slosh_kinematics *k, size_t n) {
     for (size_t i = 0; i < n && d->mass_state != 2; i++) {
         if (!k[i].enabled) {
-            d->mass_state = 2;
+            if (f2u(k[i].mass) != 0u) d->mass_state = 2;
         } else if (d->mass_state == 0) {
             d->u_mass = k[i].mass;
             d->mass_state = 1;
@@ -726,7 +728,10 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     const bool any_hook = hooks && (hooks->after_particle_sort || hooks->run_p2g || hooks->run_g2p || hooks->after_p2g ||
                                     hooks->after_grid_update || hooks->after_g2p || hooks->after_particles_update);
     // Hooks may write particle masses through their device views.
-    if (any_hook) d->mass_state = 2;
+    if (any_hook) {
+        d->masses_exposed = true;
+        d->mass_state = 2;
+    }
     if (any_hook && d->nbodies > 0 && !d->coll_cache) {
         // Hooks may move bodies: from now on keep the reference's per-node cache.
         for (int i = 0; i < 2; i++)
@@ -2023,12 +2028,18 @@ int slosh_write_kinematics(slosh_data *d, slosh_stream stream, const slosh_kinem
         d->has_force = 1;
         invalidate_graphs(d);
     }
+    // The masses and enabled flags were all rewritten.
+    if (!d->masses_exposed) {
+        d->mass_state = 0;
+        note_masses(d, kin, n);
+    }
     return SLOSH_OK;
 }
 
 int slosh_device_view_get(slosh_data *d, slosh_device_view *out) {
     if (!d || !out) return fail(SLOSH_E_INVALID, "null argument");
-    d->mass_state = 2;  // the caller may write particle masses through the view
+    d->masses_exposed = true;  // the caller may write particle masses through the view
+    d->mass_state = 2;
     PBufs &p = d->pb[d->cur_p];
     out->num_particles = d->n;
     out->particle_capacity = d->pcap;

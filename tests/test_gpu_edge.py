@@ -11,6 +11,8 @@
   round(x / h) - 1 uses round-half-to-even (WGSL round, grid.slang:346-355), and
   positions exactly on block boundaries; the block sets and per-block counts must
   be bit-exact.
+* Masses rewritten through slosh_write_kinematics (non-uniform, some particles
+  disabled) take effect in the next substep.
 """
 import numpy as np
 import pytest
@@ -111,3 +113,36 @@ def test_rounding_ties_and_block_boundaries(gpu_pipeline):
     print("\n[ties] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()))
     for key in ("x", "v", "F"):
         assert errs[key] <= TOL[1], key
+
+
+def test_written_masses_take_effect(gpu_pipeline):
+    """A data created with one mass for every particle reads masses from its
+    arguments; rewriting the kinematics with other masses (and some particles
+    disabled) must switch that off: one more substep matches an oracle restarted
+    from the written state."""
+    import copy
+    pl, st = gpu_pipeline
+    sc = scenes.stir(scenes.elastic_cube(n_side=12), 21)
+    data = P.MpmData.from_scene(pl, sc)
+    pl.launch_step(data, st)
+    kin = data.kinematics(st)
+    kin["mass"][::3] *= 2.0
+    kin["enabled"][1::11] = 0
+    data.write_kinematics(kin, st)
+    rs = copy.copy(sc)
+    pos = data.positions(st)
+    pos[:, 3] = 0.0
+    rs.particles = {"pos4": pos, "kin": data.kinematics(st), "def_grad": data.def_grad(st),
+                    "props": data.properties(st), "models": data.models(st)}
+    assert np.array_equal(rs.particles["kin"]["mass"], kin["mass"])
+    orc = O.Oracle(rs)
+    pl.launch_step(data, st)
+    data.check_errors(st)
+    orc.step(1)
+    errs = state_errors((data.positions(st), data.kinematics(st), data.def_grad(st)),
+                        (orc.positions(), orc.kinematics(), orc.def_grad()), rs.particles["pos4"])
+    print("\n[written masses] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()))
+    assert errs["enabled_diff"] == 0
+    for k in ("x", "v", "affine", "F"):
+        assert errs[k] <= TOL[1], k
+    assert np.array_equal(data.kinematics(st)["mass"], orc.kinematics()["mass"])
