@@ -156,10 +156,24 @@ struct PBufs {
     uint32_t *ids = nullptr;
 };
 
+// Creation-time records of consecutive particles of one scene. A run whose
+// particles all share one model and one property record (a scene of one
+// material) keeps that record once; `expand` turns it into one record per
+// particle before per-particle edits.
 struct HostRun {
     uint32_t scene = 0;
-    std::vector<slosh_particle_model> models;
-    std::vector<slosh_particle_properties> props;
+    size_t count = 0;
+    bool uniform = false;
+    std::vector<slosh_particle_model> recs_m;
+    std::vector<slosh_particle_properties> recs_p;
+    const slosh_particle_model &model(size_t i) const { return uniform ? recs_m[0] : recs_m[i]; }
+    const slosh_particle_properties &props(size_t i) const { return uniform ? recs_p[0] : recs_p[i]; }
+    void expand() {
+        if (!uniform) return;
+        recs_m.assign(count, recs_m[0]);
+        recs_p.assign(count, recs_p[0]);
+        uniform = false;
+    }
 };
 
 struct slosh_data {
@@ -223,11 +237,16 @@ struct slosh_data {
     uint32_t *mesh_idx = nullptr;
     size_t mesh_nv = 0, mesh_ni = 0;
     bool has_trimesh = false;
-    // Pinned host arena that appends stage particles in (reused; stage_done marks
-    // the completion of the last upload from it).
-    uint8_t *stage_arena = nullptr;
-    size_t stage_bytes = 0;
-    hipEvent_t stage_done = nullptr;
+    // Pinned host arenas that appends stage particles in, each reused once the
+    // uploads queued from it (marked by `done`) have completed.
+    struct StageArena {
+        uint8_t *ptr = nullptr;
+        size_t bytes = 0;
+        hipEvent_t done = nullptr;
+    };
+    static constexpr int MAX_STAGE_ARENAS = 4;
+    StageArena stage[MAX_STAGE_ARENAS];
+    int num_stage = 0;
     uint2 *mats = nullptr;
     int *impulses = nullptr;
     // captured substep pairs, one per starting parity
@@ -382,6 +401,16 @@ static void note_records(slosh_data *d, const slosh_particle_model *m, const slo
     d->uniform = d->uni_state == 1 ? 1u : 0u;
 }
 
+// note_records(-1) for particles [a, b) of run r.
+static void unnote_run(slosh_data *d, const HostRun &r, size_t a, size_t b) {
+    if (r.uniform) {
+        d->tag_count[tag_slot(r.recs_m[0])] -= (uint64_t)(b - a);
+        note_records(d, nullptr, nullptr, 0, -1);  // refreshes the tag mask
+    } else {
+        note_records(d, r.recs_m.data() + a, r.recs_p.data() + a, b - a, -1);
+    }
+}
+
 // Tracks whether every particle given so far has mass = enabled ? u_mass : +0
 // (bitwise), the form the fused update keeps (it zeroes the mass it disables).
 static void note_masses(slosh_data *d, const slosh_kinematics *k, size_t n) {
@@ -397,7 +426,13 @@ static void note_masses(slosh_data *d, const slosh_kinematics *k, size_t n) {
     }
 }
 
-// Appends records to the host runs (split where the scene id changes).
+static bool same_records(const slosh_particle_model &a, const slosh_particle_properties &pa,
+                         const slosh_particle_model &b, const slosh_particle_properties &pb) {
+    return std::memcmp(&a, &b, sizeof a) == 0 && std::memcmp(&pa, &pb, sizeof pa) == 0;
+}
+
+// Appends records to the host runs (split where the scene id changes). A range
+// whose records are all equal joins or starts a uniform run without copying them.
 static void host_append(slosh_data *d, const slosh_particles *p) {
     size_t i = 0;
     while (i < p->len) {
@@ -409,8 +444,19 @@ static void host_append(slosh_data *d, const slosh_particles *p) {
             d->runs.back().scene = sc;
         }
         HostRun &r = d->runs.back();
-        r.models.insert(r.models.end(), p->models + i, p->models + j);
-        r.props.insert(r.props.end(), p->properties + i, p->properties + j);
+        bool uni = true;
+        for (size_t k = i + 1; k < j && uni; k++)
+            uni = same_records(p->models[k], p->properties[k], p->models[i], p->properties[i]);
+        if (uni && r.count == 0) {
+            r.recs_m.assign(1, p->models[i]);
+            r.recs_p.assign(1, p->properties[i]);
+            r.uniform = true;
+        } else if (!(uni && r.uniform && same_records(r.recs_m[0], r.recs_p[0], p->models[i], p->properties[i]))) {
+            r.expand();
+            r.recs_m.insert(r.recs_m.end(), p->models + i, p->models + j);
+            r.recs_p.insert(r.recs_p.end(), p->properties + i, p->properties + j);
+        }
+        r.count += j - i;
         i = j;
     }
 }
@@ -418,27 +464,34 @@ static void host_append(slosh_data *d, const slosh_particles *p) {
 static std::vector<slosh_particle_model> flat_models(const slosh_data *d) {
     std::vector<slosh_particle_model> v;
     v.reserve(d->n);
-    for (const auto &r : d->runs) v.insert(v.end(), r.models.begin(), r.models.end());
+    for (const auto &r : d->runs) {
+        if (r.uniform) v.insert(v.end(), r.count, r.recs_m[0]);
+        else v.insert(v.end(), r.recs_m.begin(), r.recs_m.end());
+    }
     return v;
 }
 static std::vector<slosh_particle_properties> flat_props(const slosh_data *d) {
     std::vector<slosh_particle_properties> v;
     v.reserve(d->n);
-    for (const auto &r : d->runs) v.insert(v.end(), r.props.begin(), r.props.end());
+    for (const auto &r : d->runs) {
+        if (r.uniform) v.insert(v.end(), r.count, r.recs_p[0]);
+        else v.insert(v.end(), r.recs_p.begin(), r.recs_p.end());
+    }
     return v;
 }
 static std::vector<uint32_t> flat_scenes(const slosh_data *d) {
     std::vector<uint32_t> v;
     v.reserve(d->n);
-    for (const auto &r : d->runs) v.insert(v.end(), r.models.size(), r.scene);
+    for (const auto &r : d->runs) v.insert(v.end(), r.count, r.scene);
     return v;
 }
 // Writes models (original-id order) back into the runs.
 static void set_models(slosh_data *d, const std::vector<slosh_particle_model> &m) {
     size_t o = 0;
     for (auto &r : d->runs) {
-        std::copy(m.begin() + o, m.begin() + o + r.models.size(), r.models.begin());
-        o += r.models.size();
+        r.expand();
+        std::copy(m.begin() + o, m.begin() + o + r.count, r.recs_m.begin());
+        o += r.count;
     }
 }
 
@@ -921,11 +974,13 @@ static void free_data(slosh_data *d) {
     dfree(d->mesh_vtx);
     dfree(d->mesh_idx);
     dfree(d->dt_bound);
-    if (d->stage_done) {
-        (void)hipEventSynchronize(d->stage_done);
-        (void)hipEventDestroy(d->stage_done);
+    for (int i = 0; i < d->num_stage; i++) {
+        if (d->stage[i].done) {
+            (void)hipEventSynchronize(d->stage[i].done);
+            (void)hipEventDestroy(d->stage[i].done);
+        }
+        if (d->stage[i].ptr) (void)hipHostFree(d->stage[i].ptr);
     }
-    if (d->stage_arena) (void)hipHostFree(d->stage_arena);
     if (d->capture_stream) (void)hipStreamDestroy(d->capture_stream);
     delete d;
 }
@@ -1376,8 +1431,9 @@ int slosh_read_properties(slosh_data *d, slosh_stream stream, slosh_particle_pro
     if (!d || (!out && d->n)) return fail(SLOSH_E_INVALID, "null argument");
     size_t o = 0;
     for (const auto &r : d->runs) {
-        std::copy(r.props.begin(), r.props.end(), out + o);
-        o += r.props.size();
+        if (r.uniform) std::fill(out + o, out + o + r.count, r.recs_p[0]);
+        else std::copy(r.recs_p.begin(), r.recs_p.end(), out + o);
+        o += r.count;
     }
     return SLOSH_OK;
 }
@@ -1387,8 +1443,8 @@ int slosh_read_scene_ids(slosh_data *d, slosh_stream stream, uint32_t *out) {
     if (!d || (!out && d->n)) return fail(SLOSH_E_INVALID, "null argument");
     size_t o = 0;
     for (const auto &r : d->runs) {
-        std::fill(out + o, out + o + r.models.size(), r.scene);
-        o += r.models.size();
+        std::fill(out + o, out + o + r.count, r.scene);
+        o += r.count;
     }
     return SLOSH_OK;
 }
@@ -1610,6 +1666,42 @@ static void concat_staging(Staging &a, const Staging &b) {
     a = std::move(r);
 }
 
+// A pinned arena of at least `need` bytes whose earlier uploads are done: an idle
+// one that fits, else an idle one grown, else a new one (up to MAX_STAGE_ARENAS),
+// else the first, once its uploads complete. Pinning is slow, so arenas grow
+// geometrically.
+static int pick_stage_arena(slosh_data *d, size_t need, slosh_data::StageArena **out) {
+    slosh_data::StageArena *fit = nullptr, *idle = nullptr;
+    for (int i = 0; i < d->num_stage; i++) {
+        slosh_data::StageArena &a = d->stage[i];
+        bool ready = true;
+        if (a.done) {
+            ready = hipEventQuery(a.done) == hipSuccess;
+            (void)hipGetLastError();  // a not-ready query must not linger as the last error
+        }
+        if (!ready) continue;
+        if (a.bytes >= need && (!fit || a.bytes < fit->bytes)) fit = &a;
+        if (!idle) idle = &a;
+    }
+    slosh_data::StageArena *a = fit ? fit : idle;
+    if (!a && d->num_stage < slosh_data::MAX_STAGE_ARENAS) a = &d->stage[d->num_stage++];
+    if (!a) {
+        a = &d->stage[0];
+        HIP_TRY(hipEventSynchronize(a->done));
+    }
+    if (a->bytes < need) {
+        const size_t bytes = std::max(need, a->bytes + a->bytes / 2);
+        if (a->ptr) HIP_TRY(hipHostFree(a->ptr));
+        a->ptr = nullptr;
+        a->bytes = 0;
+        HIP_TRY(hipHostMalloc((void **)&a->ptr, bytes, hipHostMallocDefault));
+        a->bytes = bytes;
+    }
+    if (!a->done) HIP_TRY(hipEventCreateWithFlags(&a->done, hipEventDisableTiming));
+    *out = a;
+    return SLOSH_OK;
+}
+
 int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_particles *particles) {
     if (!d) return fail(SLOSH_E_INVALID, "null data");
     TRY(validate_particles(particles));
@@ -1622,33 +1714,23 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
     for (size_t i = 0; i < particles->len && !need_sand; i++) need_sand = model_is_sand(particles->models[i]);
     need_sand = need_sand && !d->has_sand;
     if (d->n + particles->len <= d->pcap && !need_sand) {
-        // Into the free tail: stage in the pinned arena and do the host bookkeeping
+        // Into the free tail: stage in a pinned arena and do the host bookkeeping
         // while substeps already queued on `stream` run, then queue the copies on
-        // `stream` (the next substeps follow them) without waiting for them. The
-        // wait for the queued substeps is for the graphs they replay, released
-        // only once they are done.
+        // `stream` (the next substeps follow them) without waiting for them. Only
+        // captured graphs need the queued substeps to finish first (they are
+        // released); without graphs the host goes on at once.
         const size_t need = PinnedStaging::WORDS * 4 * particles->len;
-        if (d->stage_done) HIP_TRY(hipEventSynchronize(d->stage_done));
-        if (need > d->stage_bytes) {
-            // Pinning is slow: grow geometrically so a stream of growing appends
-            // reallocates a few times only.
-            const size_t bytes = std::max(need, d->stage_bytes + d->stage_bytes / 2);
-            if (d->stage_arena) HIP_TRY(hipHostFree(d->stage_arena));
-            d->stage_arena = nullptr;
-            d->stage_bytes = 0;
-            HIP_TRY(hipHostMalloc((void **)&d->stage_arena, bytes, hipHostMallocDefault));
-            d->stage_bytes = bytes;
-        }
-        if (!d->stage_done) HIP_TRY(hipEventCreateWithFlags(&d->stage_done, hipEventDisableTiming));
-        PinnedStaging s(d->stage_arena, particles->len);
+        slosh_data::StageArena *arena = nullptr;
+        TRY(pick_stage_arena(d, need, &arena));
+        PinnedStaging s(arena->ptr, particles->len);
         TRY(stage_particles(particles, (uint32_t)d->n, s, &any_force, &any_sand));
         host_append(d, particles);
         note_records(d, particles->models, particles->properties, particles->len, 1);
         note_masses(d, particles->kinematics, particles->len);
-        HIP_TRY(hipStreamSynchronize(st));
+        if (d->graph[0] || d->graph[1] || d->graph[2] || d->graph[3]) HIP_TRY(hipStreamSynchronize(st));
         invalidate_graphs(d);
         TRY(upload_staging(d, s, d->cur_p, d->n, d->n, st, false));
-        HIP_TRY(hipEventRecord(d->stage_done, st));
+        HIP_TRY(hipEventRecord(arena->done, st));
         d->n += particles->len;
         if (any_force) d->has_force = 1;
         return SLOSH_OK;
@@ -1723,10 +1805,10 @@ static int device_remove(slosh_data *d, hipStream_t st, uint32_t mode, const std
     std::vector<HostRun> kept;
     size_t off = 0, m = 0;
     for (auto &r : d->runs) {
-        const size_t len = r.models.size();
+        const size_t len = r.count;
         if (mode == 0) {
             if (dead_host[r.scene]) {
-                note_records(d, r.models.data(), r.props.data(), len, -1);
+                unnote_run(d, r, 0, len);
             } else {
                 m += len;
                 kept.push_back(std::move(r));
@@ -1734,16 +1816,24 @@ static int device_remove(slosh_data *d, hipStream_t st, uint32_t mode, const std
         } else {
             const size_t a0 = std::max(lo, off), a1 = std::min(hi, off + len);
             if (a0 < a1) {
-                note_records(d, r.models.data() + (a0 - off), r.props.data() + (a0 - off), a1 - a0, -1);
-                r.models.erase(r.models.begin() + (a0 - off), r.models.begin() + (a1 - off));
-                r.props.erase(r.props.begin() + (a0 - off), r.props.begin() + (a1 - off));
+                unnote_run(d, r, a0 - off, a1 - off);
+                if (!r.uniform) {
+                    r.recs_m.erase(r.recs_m.begin() + (a0 - off), r.recs_m.begin() + (a1 - off));
+                    r.recs_p.erase(r.recs_p.begin() + (a0 - off), r.recs_p.begin() + (a1 - off));
+                }
+                r.count -= a1 - a0;
             }
-            if (!r.models.empty()) {
-                m += r.models.size();
+            if (r.count > 0) {
+                m += r.count;
                 if (!kept.empty() && kept.back().scene == r.scene) {
                     HostRun &k = kept.back();
-                    k.models.insert(k.models.end(), r.models.begin(), r.models.end());
-                    k.props.insert(k.props.end(), r.props.begin(), r.props.end());
+                    if (!(k.uniform && r.uniform && same_records(k.recs_m[0], k.recs_p[0], r.recs_m[0], r.recs_p[0]))) {
+                        k.expand();
+                        r.expand();
+                        k.recs_m.insert(k.recs_m.end(), r.recs_m.begin(), r.recs_m.end());
+                        k.recs_p.insert(k.recs_p.end(), r.recs_p.begin(), r.recs_p.end());
+                    }
+                    k.count += r.count;
                 } else {
                     kept.push_back(std::move(r));
                 }
@@ -1774,7 +1864,7 @@ int slosh_remove_scenes(slosh_data *d, slosh_stream stream, const uint32_t *scen
     }
     size_t cnt = 0;
     for (const auto &r : d->runs)
-        if (dead[r.scene]) cnt += r.models.size();
+        if (dead[r.scene]) cnt += r.count;
     if (removed) *removed = cnt;
     if (cnt == 0) return SLOSH_OK;
     return device_remove(d, (hipStream_t)stream, 0, dead, 0, 0);
@@ -1788,7 +1878,7 @@ size_t slosh_scene_size(const slosh_data *d, uint32_t scene_id) {
     if (!d) return 0;
     size_t c = 0;
     for (const auto &r : d->runs)
-        if (r.scene == scene_id) c += r.models.size();
+        if (r.scene == scene_id) c += r.count;
     return c;
 }
 
