@@ -26,13 +26,17 @@ def small_requests(count=6):
     return reqs
 
 
-def test_each_result_same_alone_and_in_the_mix(gpu_pipeline):
+# Substeps launched one by one (appends then queue their uploads behind running
+# substeps without waiting) or replayed from captured graphs (appends wait for
+# the queued substeps, whose graphs they release).
+@pytest.mark.parametrize("use_graphs", [False, True])
+def test_each_result_same_alone_and_in_the_mix(gpu_pipeline, use_graphs):
     pl, st = gpu_pipeline
     reqs = small_requests()
     total = sum(r.n for r in reqs)
     # Capacity for about half the requests at once, so admission waits for retirements.
     srv = SV.SceneServer(pl, reqs[0].scene, hbm_budget=4e9, max_particles=max(total // 2, max(r.n for r in reqs)),
-                         grid_capacity=1 << 14, max_chunk=16, result="full", stream=st)
+                         grid_capacity=1 << 14, max_chunk=16, result="full", stream=st, use_graphs=use_graphs)
     done = srv.serve(reqs, realtime=False)
     assert len(done) == len(reqs) and srv.data.n == 0
     # The mix really overlapped: some request was admitted while another was running,
