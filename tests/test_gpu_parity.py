@@ -380,3 +380,26 @@ def test_timestep_bound_matches_oracle(gpu_pipeline, make):
     assert data.adaptive_substeps(base, 20, 20, st) == 20
     pl.launch_step(data, st)   # the new dt is used (graphs re-captured on the next replay)
     data.check_errors(st)
+
+
+def test_timestamps_time_each_pass(gpu_pipeline):
+    """GpuTimestamps' pass labels (pipeline.rs:588-757): every pass the substep runs
+    is timed once per substep, and timing does not change the results."""
+    pl, st = gpu_pipeline
+    sc = scenes.stir(scenes.elastic_cube(n_side=12), 8)
+    a = P.MpmData.from_scene(pl, sc)
+    b = P.MpmData.from_scene(pl, sc)
+    ts = P.Timestamps()
+    ts.reset()
+    for _ in range(3):
+        pl.launch_step(a, st, timestamps=ts)
+        pl.launch_step(b, st)
+    a.check_errors(st)
+    r = ts.read()
+    print("\n[timestamps] " + " ".join(f"{k}={v[0] / max(v[1], 1):.4f}ms x{v[1]}" for k, v in r.items()))
+    for name in ("grid_sort", "p2g", "grid_update", "g2p"):
+        ms, count = r[name]
+        assert count == 3 and ms > 0.0, name
+    from tests.parity import rel_l2
+    assert rel_l2(a.kinematics(st)["velocity"], b.kinematics(st)["velocity"]) < 1e-5
+    assert rel_l2(a.positions(st)[:, :3], b.positions(st)[:, :3]) < 1e-6
