@@ -170,8 +170,10 @@ struct HostRun {
     const slosh_particle_properties &props(size_t i) const { return uniform ? recs_p[0] : recs_p[i]; }
     void expand() {
         if (!uniform) return;
-        recs_m.assign(count, recs_m[0]);
-        recs_p.assign(count, recs_p[0]);
+        const slosh_particle_model m0 = recs_m[0];  // assign() must not take a reference into the vector
+        const slosh_particle_properties p0 = recs_p[0];
+        recs_m.assign(count, m0);
+        recs_p.assign(count, p0);
         uniform = false;
     }
 };
