@@ -129,3 +129,26 @@ def test_bvh_layout_and_pseudo_normals():
     assert np.min(np.einsum("ij,ij->i", vn, v / np.linalg.norm(v, axis=1, keepdims=True))) > 0.999
     mid = (v[t] + v[np.roll(t, -1, axis=1)]) / 2
     assert np.min(np.einsum("ijk,ijk->ij", en, mid)) > 0
+
+
+def test_builders_reject_malformed_input():
+    import pytest
+    buf = TM.CollisionBuffers()
+    v, t = TM.box_mesh((0.1, 0.1, 0.1))
+    with pytest.raises(ValueError):
+        TM.convert(v, np.zeros((0, 3), np.uint32), buf)  # no triangles
+    with pytest.raises(ValueError):
+        TM.convert(v, t + 8, buf)  # vertex index out of range
+    assert buf.nv == 0 and buf.ni == 0  # nothing appended by a rejected mesh
+    with pytest.raises(ValueError):
+        TM.heightfield(np.zeros(5), (1, 1, 1))  # not a grid
+    with pytest.raises(ValueError):
+        TM.heightfield(np.zeros((1, 4)), (1, 1, 1))  # fewer than 2 samples along z
+    # Two meshes share the buffers: the second's roots follow the first's entries.
+    a = TM.convert(v, t, buf)
+    b = TM.convert(*TM.icosphere(0.2, 0), buf)
+    ra, rb = np.array(a["a"]).view(np.uint32), np.array(b["a"]).view(np.uint32)
+    assert ra[0] == 0 and ra[1] == 0
+    assert rb[0] == 2 * ra[2] + 2 * len(v) + 3 * len(t)
+    assert rb[1] == 3 * ra[2] + 3 * len(t)
+    assert buf.vertex_array().shape[0] == buf.nv and buf.index_array().size == buf.ni
