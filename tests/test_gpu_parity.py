@@ -403,3 +403,20 @@ def test_timestamps_time_each_pass(gpu_pipeline):
     from tests.parity import rel_l2
     assert rel_l2(a.kinematics(st)["velocity"], b.kinematics(st)["velocity"]) < 1e-5
     assert rel_l2(a.positions(st)[:, :3], b.positions(st)[:, :3]) < 1e-6
+
+
+def test_autotune_with_trimesh_colliders(gpu_pipeline):
+    """The autotuner's private copy carries the collision mesh; the data does not
+    advance and its next substep still matches the oracle."""
+    pl, st = gpu_pipeline
+    sc = scenes.trimesh_test_scene()
+    data = P.MpmData.from_scene(pl, sc)
+    x0 = data.positions(st).copy()
+    tuning = data.autotune(pl, st, trial_steps=2)
+    assert np.array_equal(data.positions(st), x0)
+    assert tuning["p2g_chunk"] in (192, 256, 384, 512)
+    pl.launch_step(data, st)
+    data.check_errors(st)
+    orc = O.Oracle(sc)
+    orc.step(1)
+    check_state(sc, data, orc, st, 1)
