@@ -110,7 +110,9 @@ def run_plain(args):
     data = P.MpmData.from_scene(pl, scene)
     # warm-up (also sorts particle storage by block)
     pl.launch_step(data, st)
-    tuning = data.autotune(pl, st, trial_steps=10) if args.autotune else data.tuning
+    # 40-substep trials (~4 ms each): shorter ones let timing noise pick among
+    # launch shapes within a few per cent of each other.
+    tuning = data.autotune(pl, st, trial_steps=40) if args.autotune else data.tuning
     pl.launch_steps_graph(data, max(args.warmup - 1, 0), st)
     st.synchronize()
     data.check_errors(st)
