@@ -32,6 +32,7 @@ extern "C" {
 #define SLOSH_E_HOOK -5          /* a hook returned an error */
 #define SLOSH_E_UNSUPPORTED -6   /* input the library does not implement (e.g. trimesh colliders) */
 #define SLOSH_E_NO_KERNELS -7    /* pipeline built without builtin transfers and no hook handled the pass */
+#define SLOSH_E_KEY_RANGE -8     /* a block coordinate outside the block-key space (reference: silent wrap) */
 
 /* Material tags of GpuParticleModel (src/solver/particle_model.rs:89-100). */
 #define SLOSH_MODEL_ELASTIC_LINEAR 0u
@@ -53,6 +54,14 @@ extern "C" {
 
 #define SLOSH_MAX_COLLIDERS 16 /* GpuMaterials::new assert, src/solver/boundary_condition.rs:77-80 */
 #define SLOSH_MAX_SCENES 16383 /* batched key space: 14-bit scene id */
+/* Block-key range. Single scene (pack_key, grid.slang:83-90): block coordinates
+ * x, z in [-1023, 1023], y in [-511, 511]. Batched: 6 bits per axis, every block
+ * coordinate in [-16, 46] (with dx = 1/128, positions in about [-0.5, 1.49]).
+ * The "+1" neighbour of every block then has an exact key too. Creation and
+ * append reject particles outside it (SLOSH_E_KEY_RANGE); a particle that moves
+ * out of it is reported by slosh_check_errors (SLOSH_E_KEY_RANGE). */
+#define SLOSH_BATCHED_BLOCK_MIN (-16)
+#define SLOSH_BATCHED_BLOCK_MAX 46
 
 typedef void *slosh_stream; /* hipStream_t; NULL = the default stream */
 typedef struct slosh_pipeline slosh_pipeline;
@@ -257,8 +266,9 @@ int slosh_timestep_bound(slosh_data *data, slosh_stream stream, float *out_dt, u
 int slosh_adaptive_substeps(slosh_data *data, slosh_stream stream, float base_dt, uint32_t min_substeps,
                             uint32_t max_substeps, uint32_t *out_substeps);
 
-/* Synchronizes `stream`, returns SLOSH_E_GRID_OVERFLOW if any substep since the
- * last call overflowed the sparse-grid hash map, and clears the flag. */
+/* Synchronizes `stream`, returns SLOSH_E_KEY_RANGE if any substep since the last
+ * call saw a particle outside the block-key range, else SLOSH_E_GRID_OVERFLOW if
+ * any overflowed the sparse-grid hash map, and clears the flags. */
 int slosh_check_errors(slosh_data *data, slosh_stream stream);
 
 /* ---- Readback (staging copies of src_testbed/step.rs:271-334), in the

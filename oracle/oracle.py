@@ -34,6 +34,7 @@ def lib():
         l.so_step.argtypes = [vp]
         l.so_step.restype = C.c_int
         l.so_set_threads.argtypes = [C.c_int]
+        l.so_set_p2g_gather.argtypes = [C.c_int]
         for name in ("so_read_positions", "so_read_kinematics", "so_read_def_grad", "so_read_models",
                      "so_read_blocks", "so_read_nodes"):
             getattr(l, name).argtypes = [vp, vp]
@@ -143,6 +144,12 @@ class Oracle:
         if nb:
             lib().so_read_nodes(self._s, _p(out))
         return out
+
+
+def set_p2g_gather(gather):
+    """P2G formulation of every oracle in the process: the shader's per-node gather
+    (True) or the bit-identical block-local scatter (False, the default)."""
+    lib().so_set_p2g_gather(1 if gather else 0)
 
 
 def svd3(a, f64=True):

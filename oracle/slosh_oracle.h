@@ -86,6 +86,9 @@ void so_destroy(so_state *s);
 /* One substep. Returns 0, or -1 on hash-map overflow. */
 int so_step(so_state *s);
 void so_set_threads(int nthreads);
+/* P2G formulation (process-wide): 0 = block-local scatter (default), 1 = the
+ * shader's per-node gather (p2g_scatter_style.slang:41-155). Bit-identical. */
+void so_set_p2g_gather(int gather);
 
 void so_read_positions(const so_state *s, float *out4);
 void so_read_kinematics(const so_state *s, so_kinematics *out);

@@ -7,7 +7,7 @@ LIB_PATH = os.path.join(HERE, "libslosh_hip.so")
 
 SLOSH_OK = 0
 ERRORS = {-1: "SLOSH_E_INVALID", -2: "SLOSH_E_HIP", -3: "SLOSH_E_NO_MEMORY", -4: "SLOSH_E_GRID_OVERFLOW",
-          -5: "SLOSH_E_HOOK", -6: "SLOSH_E_UNSUPPORTED", -7: "SLOSH_E_NO_KERNELS"}
+          -5: "SLOSH_E_HOOK", -6: "SLOSH_E_UNSUPPORTED", -7: "SLOSH_E_NO_KERNELS", -8: "SLOSH_E_KEY_RANGE"}
 
 # Every symbol include/slosh_hip.h declares.
 EXPORTS = [

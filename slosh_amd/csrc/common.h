@@ -23,6 +23,7 @@
 #define CTR_ERRORS 2
 #define ERR_HASH_FULL 1u
 #define ERR_LOOKUP 2u
+#define ERR_KEY_RANGE 4u
 
 // Neighbour table per block (16 words): [0..6] the +1 neighbours in the
 // reference's associated-block order (grid.slang:371-379, offsets 1..7),
@@ -72,6 +73,15 @@ __device__ __host__ inline uint32_t pack_key_single(int x, int y, int z) {
 __device__ __host__ inline uint32_t pack_key_batched(uint32_t scene, int x, int y, int z) {
     return (scene << 18) | (((uint32_t)(z + 16) & 63u) << 12) | (((uint32_t)(y + 16) & 63u) << 6) |
            ((uint32_t)(x + 16) & 63u);
+}
+
+// Block coordinates whose key and whose "+1" neighbours' keys are exact: batched
+// keys hold 6 bits per axis with offset 16 ([-16, 47]), single-scene keys 11 / 10
+// / 11 bits with offsets 0x3ff / 0x1ff / 0x3ff (grid.slang:83-90). A block outside
+// this range would alias another block of the same key space.
+__device__ __host__ inline bool block_in_key_range(uint32_t batched, int x, int y, int z) {
+    if (batched) return x >= -16 && x <= 46 && y >= -16 && y <= 46 && z >= -16 && z <= 46;
+    return x >= -1023 && x <= 1023 && y >= -511 && y <= 511 && z >= -1023 && z <= 1023;
 }
 
 __device__ __host__ inline uint32_t pack_key(uint32_t batched, uint32_t scene, int x, int y, int z) {

@@ -123,7 +123,17 @@ __device__ inline void count_particle(uint32_t i, const float4 *__restrict__ pos
     const bool valid = i < n;
     const uint32_t lane = lane_id();
     uint32_t cell = 0u, key = 0u;
-    if (valid) key = particle_key(pos[i], h, batched, &cell);
+    bool in_range = true;
+    if (valid) {
+        const float4 p = pos[i];
+        int bx, by, bz;
+        assoc_block_cell(h, p.x, p.y, p.z, &bx, &by, &bz, &cell);
+        key = pack_key(batched, __float_as_uint(p.w) >> 1, bx, by, bz);
+        in_range = block_in_key_range(batched, bx, by, bz);
+    }
+    // A block outside the key space aliases another one: reported, once per wave.
+    if (__ballot(!in_range) && lane == (uint32_t)(__ffsll((unsigned long long)__ballot(!in_range)) - 1))
+        atomicOr(&counters[CTR_ERRORS], ERR_KEY_RANGE);
     // Hash lookup by the head of each run of equal block keys, all runs at once.
     const uint64_t bheads = run_heads(key, valid);
     uint32_t hv = SLOSH_NONE;

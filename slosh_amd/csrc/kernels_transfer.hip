@@ -79,8 +79,11 @@ __device__ inline void mul3_tn(const float *a, const float *b, float *r) {
 // symmetric Schur rotations), eigenvalues sorted descending, then a Givens QR
 // of A V (the layout of McAdams et al. 2011). U and V are rotations,
 // sig0 >= sig1 >= |sig2| and sign(sig2) = sign(det A). T = float or double.
+// Evaluated without multiply-add contraction, operation by operation as written,
+// so that the FP32 instance rounds like a plain IEEE evaluation of the same steps.
 template <typename T>
 __device__ void svd3(const T *a, T *u, T *sig, T *v) {
+#pragma clang fp contract(off)
     const T tol = sizeof(T) == 4 ? (T)1.0e-14 : (T)1.0e-30;
     T s[3][3];
 #pragma unroll
@@ -184,6 +187,7 @@ __device__ void svd3(const T *a, T *u, T *sig, T *v) {
 // U diag(s) V^T
 template <typename T>
 __device__ inline void svd_recompose(const T *u, const T *s, const T *v, T *r) {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -227,6 +231,7 @@ __device__ inline void fc_stress(float lambda, float mu, const float *f, float *
 // drucker_prager.slang:144-200: returns the projected deformation gradient in f,
 // updates state = {plastic det, hardening, log volume gain}.
 __device__ inline void dp_project(float *state, const float4 hp, float lambda, float mu, float *f) {
+#pragma clang fp contract(off)
     if (lambda == 0.0f) return;
     float u[9], s[3], v[9];
     svd3<float>(f, u, s, v);

@@ -178,6 +178,18 @@ struct HostRun {
     }
 };
 
+// A user-captured substep sequence. It is registered with its data: destroying
+// or mutating the data retires the executable, so a launch can never replay
+// kernels over freed or reallocated buffers.
+struct slosh_graph {
+    slosh_data *data = nullptr;  // null once the data was destroyed
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;   // recorded after each launch
+    uint64_t generation = 0;
+    int cur_p = 0, cur_g = 0;
+    StepParams params;           // step parameters the substeps were captured with
+};
+
 struct slosh_data {
     slosh_pipeline *pipeline = nullptr;
     size_t n = 0, pcap = 0;
@@ -251,8 +263,13 @@ struct slosh_data {
     int num_stage = 0;
     uint2 *mats = nullptr;
     int *impulses = nullptr;
-    // captured substep pairs, one per starting parity
+    // Captured substep pairs, one per starting parity, each with the step
+    // parameters it was captured with (a replay under other parameters is
+    // recaptured) and an event recorded after its last launch.
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    StepParams graph_params[4];
+    hipEvent_t graph_done[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<slosh_graph *> user_graphs;  // live slosh_graph objects of this data
     uint64_t generation = 0;  // bumped by every mutation; user graphs check it
     hipStream_t capture_stream = nullptr;
 };
@@ -291,14 +308,37 @@ static uint32_t f2u(float f) {
 // Two alternating sets of per-substep counters, one per hash table.
 static uint32_t *ctr_set(slosh_data *d, int g) { return d->counters + 8 * g; }
 
+// Releases a graph executable once its last launch has completed: a replay may
+// still be running on the GPU, and the runtime frees the executable's kernel
+// arguments with it.
+static void retire_exec(hipGraphExec_t &exec, hipEvent_t done) {
+    if (!exec) return;
+    if (done) (void)hipEventSynchronize(done);
+    (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+}
+
+// Records `*ev` (created on first use) on `st` after a graph launch, unless `st`
+// is being captured (then the launch is a node of the caller's graph).
+static void mark_launch(hipEvent_t *ev, hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    if (!*ev && hipEventCreateWithFlags(ev, hipEventDisableTiming) != hipSuccess) {
+        *ev = nullptr;
+        (void)hipStreamSynchronize(st);  // no event: make the launch complete instead
+        return;
+    }
+    (void)hipEventRecord(*ev, st);
+}
+
+// Every change to the state a captured substep reads from its kernel arguments
+// (buffers, counts, step parameters, uniform records, mass tracking, collision
+// cache) goes through here: cached pairs and user graphs are retired.
 static void invalidate_graphs(slosh_data *d) {
     d->next_primed = false;
     d->generation++;
-    for (int i = 0; i < 4; i++)
-        if (d->graph[i]) {
-            (void)hipGraphExecDestroy(d->graph[i]);
-            d->graph[i] = nullptr;
-        }
+    for (int i = 0; i < 4; i++) retire_exec(d->graph[i], d->graph_done[i]);
+    for (slosh_graph *g : d->user_graphs) retire_exec(g->exec, g->done);
 }
 
 static int alloc_particle_buffers(slosh_data *d, size_t pcap, bool sand) {
@@ -701,7 +741,8 @@ static int download_current(slosh_data *d, hipStream_t st, Snapshot &s) {
 }
 
 static StepParams make_params(const slosh_data *d) {
-    StepParams P{};
+    StepParams P;
+    std::memset(&P, 0, sizeof P);  // padding included: captured parameter sets compare bytewise
     P.gravity[0] = d->params.gravity[0];
     P.gravity[1] = d->params.gravity[1];
     P.gravity[2] = d->params.gravity[2];
@@ -782,10 +823,13 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     const slosh_kernels &K = pl->kernels;
     const bool any_hook = hooks && (hooks->after_particle_sort || hooks->run_p2g || hooks->run_g2p || hooks->after_p2g ||
                                     hooks->after_grid_update || hooks->after_g2p || hooks->after_particles_update);
-    // Hooks may write particle masses through their device views.
-    if (any_hook) {
+    // Hooks may write particle masses through their device views. Captured
+    // substeps carry the mass tracking and the collision-cache switch in their
+    // arguments, so changing either retires them.
+    if (any_hook && (!d->masses_exposed || d->mass_state != 2)) {
         d->masses_exposed = true;
         d->mass_state = 2;
+        invalidate_graphs(d);
     }
     if (any_hook && d->nbodies > 0 && !d->coll_cache) {
         // Hooks may move bodies: from now on keep the reference's per-node cache.
@@ -793,6 +837,7 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
             if (!d->coll[i]) TRY(dalloc(&d->coll[i], 2 * (size_t)d->cap * NODES_PER_BLOCK));
         d->coll_cache = 1;
         d->coll_prev_valid = 0;
+        invalidate_graphs(d);
     }
     // GpuGrid::swap_buffers (grid.rs:513-517): the table written last substep becomes "previous".
     const int prev_g = d->cur_g;
@@ -949,6 +994,10 @@ void slosh_pipeline_destroy(slosh_pipeline *p) { delete p; }
 
 static void free_data(slosh_data *d) {
     invalidate_graphs(d);
+    for (slosh_graph *g : d->user_graphs) g->data = nullptr;  // their launches now fail cleanly
+    d->user_graphs.clear();
+    for (int i = 0; i < 4; i++)
+        if (d->graph_done[i]) (void)hipEventDestroy(d->graph_done[i]);
     free_particle_buffers(d);
     for (int i = 0; i < 2; i++) {
         dfree(d->keys[i]);
@@ -1024,6 +1073,29 @@ static int upload_bodies(slosh_data *d) {
     return SLOSH_OK;
 }
 
+// Every particle's primary block, and its "+1" neighbours, must have exact keys
+// (block_associated_to_point, grid.slang:346-355, in the kernels' arithmetic).
+static int check_key_range(const slosh_particles *p, float h, uint32_t batched) {
+    const bool pow2 = (f2u(h) & 0x807fffffu) == 0u && f2u(h) - 0x00800000u < 0x7e000000u;
+    const float inv = 1.0f / h;
+    for (size_t i = 0; i < p->len; i++) {
+        int b[3];
+        for (int c = 0; c < 3; c++) {
+            const float x = p->positions[4 * i + c];
+            const float q = pow2 ? x * inv : x / h;
+            const float a = std::nearbyint(q) - 1.0f;  // round half to even (default rounding mode)
+            const float f = std::floor(a / 4.0f);
+            b[c] = std::isfinite(f) && std::fabs(f) < 1.0e6f ? (int)f : (1 << 24);
+        }
+        if (!block_in_key_range(batched, b[0], b[1], b[2]))
+            return fail(SLOSH_E_KEY_RANGE, "particle " + std::to_string(i) + " lies in block (" + std::to_string(b[0]) +
+                                               ", " + std::to_string(b[1]) + ", " + std::to_string(b[2]) +
+                                               ") outside the block-key range" +
+                                               (batched ? " (batched: [-16, 46] per axis)" : ""));
+    }
+    return SLOSH_OK;
+}
+
 static int validate_particles(const slosh_particles *p) {
     if (!p) return fail(SLOSH_E_INVALID, "null particles");
     if (p->len > 0 && (!p->positions || !p->kinematics || !p->def_grad || !p->properties || !p->models))
@@ -1053,6 +1125,7 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     d->pcap = std::max(particle_capacity, particles->len);
     if (d->pcap == 0) d->pcap = 1;
     d->batched = particles->scene_ids != nullptr;
+    TRY(check_key_range(particles, cell_width, d->batched));
     bool any_force = false, any_sand = false;
     Staging s;
     TRY(stage_particles(particles, 0, s, &any_force, &any_sand));
@@ -1145,14 +1218,30 @@ int slosh_launch_step(slosh_pipeline *pipeline, slosh_data *d, slosh_stream stre
 int slosh_launch_steps_graph(slosh_pipeline *pipeline, slosh_data *d, slosh_stream stream, uint32_t count) {
     if (!pipeline || !d) return fail(SLOSH_E_INVALID, "null pipeline or data");
     hipStream_t st = (hipStream_t)stream;
+    bool launched[4] = {false, false, false, false};
+    auto mark = [&]() {
+        for (int i = 0; i < 4; i++)
+            if (launched[i]) mark_launch(&d->graph_done[i], st);
+    };
     while (count >= 2) {
         // The substep alternates particle buffers and hash tables, so a pair of
         // substeps returns to the same parity; one graph per starting parity.
         const int slot = (d->cur_p & 1) | ((d->cur_g & 1) << 1);
         if (d->has_force || (d->coll_cache && !d->coll_prev_valid) || !d->next_primed) {
-            TRY(run_step(pipeline, d, st, nullptr, nullptr, nullptr));
+            const int rc = run_step(pipeline, d, st, nullptr, nullptr, nullptr);
+            if (rc != SLOSH_OK) {
+                mark();
+                return rc;
+            }
             count--;
             continue;
+        }
+        // A pair captured under other step parameters is stale (a guard behind
+        // invalidate_graphs: every parameter a replay would reuse is compared).
+        const StepParams now = make_params(d);
+        if (d->graph[slot] && std::memcmp(&now, &d->graph_params[slot], sizeof now) != 0) {
+            retire_exec(d->graph[slot], d->graph_done[slot]);
+            launched[slot] = false;
         }
         if (!d->graph[slot]) {
             const int save_p = d->cur_p, save_g = d->cur_g;
@@ -1174,21 +1263,24 @@ int slosh_launch_steps_graph(slosh_pipeline *pipeline, slosh_data *d, slosh_stre
             if (e != hipSuccess) return fail(SLOSH_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
             e = hipGraphInstantiate(&d->graph[slot], graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) return fail(SLOSH_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            if (e != hipSuccess) {
+                d->graph[slot] = nullptr;
+                return fail(SLOSH_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            }
+            d->graph_params[slot] = now;
         }
-        HIP_TRY(hipGraphLaunch(d->graph[slot], st));
+        const hipError_t le = hipGraphLaunch(d->graph[slot], st);
+        if (le != hipSuccess) {
+            mark();
+            return fail(SLOSH_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(le));
+        }
+        launched[slot] = true;
         count -= 2;
     }
+    mark();
     if (count == 1) TRY(run_step(pipeline, d, st, nullptr, nullptr, nullptr));
     return SLOSH_OK;
 }
-
-struct slosh_graph {
-    slosh_data *data = nullptr;
-    hipGraphExec_t exec = nullptr;
-    uint64_t generation = 0;
-    int cur_p = 0, cur_g = 0;
-};
 
 int slosh_graph_capture(slosh_pipeline *pipeline, slosh_data *d, uint32_t steps, slosh_graph **out) {
     if (!pipeline || !d || !out) return fail(SLOSH_E_INVALID, "null argument");
@@ -1197,6 +1289,7 @@ int slosh_graph_capture(slosh_pipeline *pipeline, slosh_data *d, uint32_t steps,
         return fail(SLOSH_E_INVALID, "run one slosh_launch_step before capturing (state not steady)");
     *out = nullptr;
     slosh_graph *g = new slosh_graph();
+    g->params = make_params(d);
     const int save_p = d->cur_p, save_g = d->cur_g;
     const uint32_t save_force = d->has_force;
     const bool save_primed = d->next_primed;
@@ -1230,22 +1323,32 @@ int slosh_graph_capture(slosh_pipeline *pipeline, slosh_data *d, uint32_t steps,
     g->generation = d->generation;
     g->cur_p = save_p;
     g->cur_g = save_g;
+    d->user_graphs.push_back(g);
     *out = g;
     return SLOSH_OK;
 }
 
 int slosh_graph_launch(slosh_graph *g, slosh_stream stream) {
-    if (!g || !g->exec) return fail(SLOSH_E_INVALID, "null graph");
+    if (!g) return fail(SLOSH_E_INVALID, "null graph");
     slosh_data *d = g->data;
-    if (d->generation != g->generation || d->cur_p != g->cur_p || d->cur_g != g->cur_g || !d->next_primed)
+    if (!d) return fail(SLOSH_E_INVALID, "the graph's data was destroyed");
+    const StepParams now = make_params(d);
+    if (!g->exec || d->generation != g->generation || d->cur_p != g->cur_p || d->cur_g != g->cur_g ||
+        !d->next_primed || std::memcmp(&now, &g->params, sizeof now) != 0)
         return fail(SLOSH_E_INVALID, "graph no longer matches the data state (mutated or stepped an odd count)");
     HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    mark_launch(&g->done, (hipStream_t)stream);
     return SLOSH_OK;
 }
 
 void slosh_graph_destroy(slosh_graph *g) {
     if (!g) return;
-    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->data) {
+        auto &v = g->data->user_graphs;
+        v.erase(std::remove(v.begin(), v.end(), g), v.end());
+    }
+    retire_exec(g->exec, g->done);
+    if (g->done) (void)hipEventDestroy(g->done);
     delete g;
 }
 
@@ -1353,10 +1456,15 @@ int slosh_check_errors(slosh_data *d, slosh_stream stream) {
     uint32_t c[16];
     HIP_TRY(hipMemcpyAsync(c, d->counters, 64, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if ((c[CTR_ERRORS] | c[8 + CTR_ERRORS]) != 0u) {
+    const uint32_t err = c[CTR_ERRORS] | c[8 + CTR_ERRORS];
+    if (err != 0u) {
         uint32_t zero = 0;
         HIP_TRY(hipMemcpy(d->counters + CTR_ERRORS, &zero, 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d->counters + 8 + CTR_ERRORS, &zero, 4, hipMemcpyHostToDevice));
+        if (err & ERR_KEY_RANGE)
+            return fail(SLOSH_E_KEY_RANGE, d->batched ? "a particle left its scene's block-key range (batched "
+                                                        "blocks must stay in [-16, 46] per axis)"
+                                                      : "a particle left the block-key range");
         return fail(SLOSH_E_GRID_OVERFLOW, "sparse-grid hash map overflow (increase grid_capacity)");
     }
     return SLOSH_OK;
@@ -1709,6 +1817,7 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
     TRY(validate_particles(particles));
     if ((particles->scene_ids != nullptr) != (d->batched != 0))
         return fail(SLOSH_E_INVALID, "scene ids must be given iff the data is batched");
+    TRY(check_key_range(particles, d->h, d->batched));
     hipStream_t st = (hipStream_t)stream;
     if (particles->len == 0) return SLOSH_OK;
     bool any_force = false, any_sand = false;
@@ -2116,22 +2225,26 @@ int slosh_write_kinematics(slosh_data *d, slosh_stream stream, const slosh_kinem
             HIP_TRY(hipMemcpyAsync(d->fdt + k * pc, s.fdt.data() + k * n, 4 * n, hipMemcpyHostToDevice, st));
         HIP_TRY(hipStreamSynchronize(st));
     }
-    if (any_force) {
-        d->has_force = 1;
-        invalidate_graphs(d);
-    }
-    // The masses and enabled flags were all rewritten.
+    if (any_force) d->has_force = 1;
+    // The masses and enabled flags were all rewritten, and captured substeps may
+    // carry the old uniform mass in their arguments.
     if (!d->masses_exposed) {
         d->mass_state = 0;
         note_masses(d, kin, n);
     }
+    invalidate_graphs(d);
     return SLOSH_OK;
 }
 
 int slosh_device_view_get(slosh_data *d, slosh_device_view *out) {
     if (!d || !out) return fail(SLOSH_E_INVALID, "null argument");
-    d->masses_exposed = true;  // the caller may write particle masses through the view
-    d->mass_state = 2;
+    if (!d->masses_exposed || d->mass_state != 2) {
+        // The caller may write particle masses through the view: substeps captured
+        // with the uniform mass in their arguments are retired.
+        d->masses_exposed = true;
+        d->mass_state = 2;
+        invalidate_graphs(d);
+    }
     PBufs &p = d->pb[d->cur_p];
     out->num_particles = d->n;
     out->particle_capacity = d->pcap;

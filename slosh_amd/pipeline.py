@@ -156,13 +156,16 @@ class MpmPipeline:
 
 
 class Graph:
-    """`steps` substeps of `data` captured once into a HIP graph and replayed with ``launch``."""
+    """`steps` substeps of `data` captured once into a HIP graph and replayed with ``launch``.
+
+    The graph keeps its data alive; if the data is closed first, ``launch`` raises."""
 
     def __init__(self, pipeline, data, steps):
         h = C.c_void_p()
         _lib.check(_lib.load().slosh_graph_capture(pipeline.handle, data.handle, int(steps), C.byref(h)))
         self.handle = h
         self.steps = steps
+        self._data = data
 
     def launch(self, stream=None):
         _lib.check(_lib.load().slosh_graph_launch(self.handle, stream.handle if stream else None))

@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 from slosh_amd import _lib
@@ -52,3 +53,42 @@ def test_struct_sizes_match_reference_layouts():
     assert L.BLOCK_INFO.itemsize == 64
     assert C.sizeof(_lib.Particles) == 56
     assert C.sizeof(_lib.Kernels) == 24
+
+
+def _particles(pos, h=1.0 / 128):
+    from slosh_amd import models as M
+    return M.particle_arrays(np.asarray(pos, np.float32), np.float32(h / 4), 1000.0, M.elastic_neo_hookean(1e5, 0.3))
+
+
+def test_block_key_range_is_checked_before_any_gpu_work():
+    """Creation rejects particles whose block (or its "+1" neighbour) has no exact
+    key: batched blocks must lie in [-16, 46] per axis (6 bits, offset 16),
+    single-scene blocks in x, z [-1023, 1023], y [-511, 511]. The check runs on
+    the host before any device allocation, so it needs no GPU."""
+    from slosh_amd import layouts as L
+    from slosh_amd import pipeline as P
+    h = 1.0 / 128
+    pl = P.MpmPipeline()
+
+    def create(pos, scene_ids):
+        parts = _particles(pos, h)
+        st, keep = P._particles_struct(parts, scene_ids)
+        prm = np.zeros(1, dtype=L.SIM_PARAMS)
+        prm["dt"] = 1e-4
+        out = C.c_void_p()
+        rc = _lib.load().slosh_data_create(pl.handle, C.c_void_p(prm.ctypes.data), C.byref(st), 0, None, 0, None,
+                                           C.c_float(h), 1024, C.byref(out))
+        return rc, _lib.load().slosh_last_error()
+
+    ids = np.zeros(2, np.uint32)
+    # x = 1.48: round(x / h) - 1 = 188, block 47 -> out of the batched range
+    rc, msg = create([[0.5, 0.5, 0.5], [1.48, 0.5, 0.5]], ids)
+    assert rc == -8 and b"particle 1" in msg and b"[-16, 46]" in msg
+    # y = -0.52: block -18
+    rc, _ = create([[0.5, -0.52, 0.5], [0.5, 0.5, 0.5]], ids)
+    assert rc == -8
+    # single scene: y blocks must stay >= -511; y = -16.0 is block -513
+    rc, _ = create([[0.5, -16.0, 0.5]], None)
+    assert rc == -8
+    rc, _ = create([[0.5, float("nan"), 0.5]], None)
+    assert rc == -8

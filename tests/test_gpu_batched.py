@@ -165,3 +165,43 @@ def test_autotune_keeps_state_and_results(gpu_pipeline):
             assert max(errs[k] for k in ("x", "v", "affine", "F")) <= tol_for(steps), (shape, sid, errs)
     with pytest.raises(Exception):
         a.set_tuning(p2g_chunk=300)
+
+
+def test_batched_key_range_is_reported(gpu_pipeline):
+    """A scene whose particles fly out of the batched key range ([-16, 46] blocks per
+    axis) is reported by check_errors, and an append outside the range is refused;
+    the other scene of the batch is unaffected by the refused append."""
+    from slosh_amd import _lib
+    from slosh_amd import models as M
+    pl, st = gpu_pipeline
+    h = 1.0 / 128
+    g = np.arange(6) * (h / 2)
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    model = M.elastic_neo_hookean(1e5, 0.3)
+
+    def scene(origin, vx):
+        parts = M.particle_arrays((lat + origin).astype(np.float32), np.float32(h / 4), 1000.0, model,
+                                  np.tile(np.float32([vx, 0.0, 0.0]), (len(lat), 1)))
+        return scenes.Scene("range", parts, 1e-4, h, 1024, [], [], gravity=(0.0, 0.0, 0.0))
+
+    # block 45 at the start (x ~ 1.40-1.42), moving at 50 m/s = 0.64 cells per substep:
+    # block 47 (x >= 1.4727) after ~10 substeps.
+    b = scenes.Batch([scene((1.40, 0.5, 0.5), 50.0), scene((0.3, 0.3, 0.3), 0.0)], grid_capacity=1024)
+    data = P.MpmData.from_batch(pl, b)
+    for _ in range(3):
+        pl.launch_step(data, st)
+    data.check_errors(st)  # still in range
+    bad = scene((1.48, 0.5, 0.5), 0.0)
+    with pytest.raises(_lib.SloshError) as e:
+        data.append(bad.particles, np.full(bad.n, 5, np.uint32), st)
+    assert e.value.code == -8
+    assert data.n == b.n
+    for _ in range(20):
+        pl.launch_step(data, st)
+    with pytest.raises(_lib.SloshError) as e:
+        data.check_errors(st)
+    assert e.value.code == -8
+    data.check_errors(st)  # the flag was cleared
+    # the resting scene is untouched by the other's escape
+    pos = data.read_scene(1, st, positions_only=True)
+    assert np.allclose(pos[:, :3], b.scenes[1].particles["pos4"][:, :3], atol=1e-6)

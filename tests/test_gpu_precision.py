@@ -60,3 +60,23 @@ def test_svd_precision_per_config(gpu_pipeline, name, steps):
     for k in keys:
         assert errs[True][k] <= TOL[steps], f"FP64 path {k}: {errs[True][k]:.3e}"
     assert np.isfinite(e32)
+
+
+@pytest.mark.parametrize("steps", [1, 100])
+@pytest.mark.parametrize("name", ["mixed", "sand_spheres", "batched_fc", "batched_dp"])
+def test_fp32_svd_path_matches_fp32_oracle(gpu_pipeline, name, steps):
+    """The plain FP32 path (slosh_set_svd_fp64(0): the reference's arithmetic) against
+    the oracle's FP32 path (svd_mode=0), same SVD algorithm and operation order, at
+    the parity gates."""
+    pl, st = gpu_pipeline
+    sc = configs()[name]
+    o32 = O.Oracle(sc, svd_mode=0)
+    o32.step(steps)
+    data = run(pl, st, sc, steps, False)
+    errs = state_errors((data.positions(st), data.kinematics(st), data.def_grad(st)),
+                        (o32.positions(), o32.kinematics(), o32.def_grad()), sc.particles["pos4"])
+    print(f"\n[{name} FP32 path vs FP32 oracle, {steps} substeps] " +
+          " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+    assert errs["enabled_diff"] == 0
+    for k in ("x", "v", "affine", "F"):
+        assert errs[k] <= TOL[steps], f"FP32 path {k}: {errs[k]:.3e}"

@@ -75,3 +75,52 @@ def test_oversized_request_is_rejected(gpu_pipeline):
                          grid_capacity=1 << 12, stream=st)
     with pytest.raises(ValueError):
         srv.submit(reqs[0])
+
+
+def test_served_stream_at_full_request_sizes(gpu_pipeline):
+    """The served configuration at its own sizes: 16 requests of the stream (N
+    log-uniform in [5e3, 2e6], 182-1429 substeps each, stirred), Poisson arrivals
+    (50 req/s on a clock of 1000 substeps per second) under the 200 GB budget.
+    Every request: finite, none disabled, masses kept bit for bit, and the same
+    result alone (its own MpmData, captured graphs) as in the mix. The requests
+    with at most 2e7 particle-substeps: also against the oracle."""
+    from oracle import oracle as O
+    pl, st = gpu_pipeline
+    arr = scenes.poisson_arrivals(50.0, 16) * 1000.0
+    reqs = []
+    for k in range(16):
+        sc, steps = scenes.served_request(k)
+        scenes.stir(sc, 600 + k)
+        reqs.append(SV.Request(k, sc, steps, arrival=float(arr[k])))
+    assert min(r.n for r in reqs) < 10_000 and max(r.n for r in reqs) > 1_500_000
+    srv = SV.SceneServer(pl, reqs[0].scene, hbm_budget=200e9, result="full", stream=st)
+    done = srv.serve(reqs, realtime=False)
+    assert len(done) == len(reqs) and srv.data.n == 0
+    assert any(a.admitted < b.admitted < a.finished for a in reqs for b in reqs if a is not b)
+    checked = 0
+    for r in reqs:
+        pos, kin, f = r.result
+        assert np.all(np.isfinite(pos[:, :3])) and np.all(np.isfinite(f))
+        assert np.all(kin["enabled"] == 1), f"request {r.rid}: disabled particles"
+        assert np.array_equal(kin["mass"], r.scene.particles["kin"]["mass"])
+        alone = P.MpmData.from_scene(pl, r.scene)
+        pl.launch_steps_graph(alone, r.substeps, st)
+        ref = (alone.positions(st), alone.kinematics(st), alone.def_grad(st))
+        alone.close()
+        errs = state_errors(r.result, ref, r.scene.particles["pos4"])
+        print(f"\n[request {r.rid} n={r.n} substeps={r.substeps}] in mix vs alone: " +
+              " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+        assert errs["enabled_diff"] == 0
+        for k in ("x", "v", "affine", "F"):
+            assert errs[k] <= tol_for(r.substeps), f"request {r.rid} {k}: {errs[k]:.3e}"
+        if r.n * r.substeps <= 2.0e7:
+            orc = O.Oracle(r.scene, threads=16)
+            orc.step(r.substeps)
+            errs = state_errors(r.result, (orc.positions(), orc.kinematics(), orc.def_grad()),
+                                r.scene.particles["pos4"])
+            print(f"  vs oracle: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+            assert errs["enabled_diff"] == 0
+            for k in ("x", "v", "affine", "F"):
+                assert errs[k] <= tol_for(r.substeps), f"request {r.rid} vs oracle {k}: {errs[k]:.3e}"
+            checked += 1
+    assert checked >= 4

@@ -441,3 +441,27 @@ def test_timestep_bound_compression_fast_particles_and_minimum():
             for i in range(sc.n) if sc.particles["kin"]["enabled"][i]]
     assert orc.timestep_bound_raw() == min(each)
     assert abs(orc.timestep_bound() - 0.5 * float(sc.cell_width) / 50.0) < 1e-6 * 0.5 * float(sc.cell_width) / 50.0
+
+
+# --- P2G formulations ----------------------------------------------------------
+
+@pytest.mark.parametrize("make", [lambda: scenes.stir(scenes.elastic_cube(n_side=12), 3),
+                                  lambda: scenes.mixed_test_scene(n_side=10),
+                                  lambda: scenes.sand_spheres(n_sand=8192, n_per_sphere=1024, n_spheres=4,
+                                                              grid_capacity=1 << 14)])
+def test_p2g_scatter_is_bit_identical_to_the_shader_gather(make):
+    """The oracle's default block-local scatter P2G against the literal per-node
+    gather of p2g_scatter_style.slang:41-155 (chunked, z-slab culled): whole
+    trajectories must agree bit for bit."""
+    sc = make()
+    states = []
+    for gather in (True, False):
+        O.set_p2g_gather(gather)
+        try:
+            o = O.Oracle(sc)
+            o.step(4)
+            states.append((o.positions(), o.kinematics(), o.def_grad(), o.nodes()))
+        finally:
+            O.set_p2g_gather(False)
+    for a, b in zip(*states):
+        assert a.tobytes() == b.tobytes()
