@@ -193,9 +193,12 @@ typedef struct {
     void *vel_grad_det;  /* float[cap]                                                  */
     void *particle_ids;  /* uint32[cap]: original particle index of each slot           */
     void *force_dt;      /* float[3][cap], indexed by current slot                      */
-    void *nodes;         /* float4[grid_capacity*64]: (velocity|momentum).xyz, mass     */
-    void *sorted_ids;    /* uint32[cap]: particle slots grouped by block and cell       */
+    void *nodes;         /* float4[grid_capacity*64] by header id: (velocity|momentum).xyz, mass */
+    void *sorted_ids;    /* uint32[cap]: particle slots grouped by block and cell (substeps  */
+                         /* that run hooks sort fully; others leave it stale)               */
     void *counters;      /* uint32[8]: [0] active blocks                                */
+    void *block_list;    /* uint32[grid_capacity]: header ids of the active blocks (a     */
+                         /* block's header id is its slot in the hash table), [0, counters[0]) */
 } slosh_device_view;
 
 /* ---- Pipeline: MpmPipeline::new / new_with_kernels (src/pipeline.rs:456-534). */

@@ -67,7 +67,8 @@ class DeviceView(C.Structure):
                 ("cell_width", C.c_float), ("positions", C.c_void_p), ("vel_mass", C.c_void_p),
                 ("affine", C.c_void_p), ("def_grad", C.c_void_p), ("vel_grad_det", C.c_void_p),
                 ("particle_ids", C.c_void_p), ("force_dt", C.c_void_p), ("nodes", C.c_void_p),
-                ("sorted_ids", C.c_void_p), ("counters", C.c_void_p)]
+                ("sorted_ids", C.c_void_p), ("counters", C.c_void_p),
+                ("block_list", C.c_void_p)]
 
 
 _lib = None

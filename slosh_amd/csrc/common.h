@@ -1,8 +1,12 @@
 // Shared device definitions of the gfx950 MPM substep.
 //
 // Grid vocabulary follows the reference (shaders/slosh/grid/grid.slang):
-// blocks of 4x4x4 nodes, a virtual block id per block, a hash map from the
-// packed block key to a dense "header id" (hid) assigned per substep.
+// blocks of 4x4x4 nodes and a virtual block id per block. A hash map from the
+// packed block key gives each active block a "header id" (hid): here the slot
+// the key occupies in the substep's table, so a lookup needs no value array
+// and no compaction pass assigns ids. Per-block arrays are indexed by slot; the
+// block list (slots of the occupied entries, one per block) gives the order the
+// per-block kernels walk and the order of particle storage.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,14 +20,39 @@
 #define NODES_PER_BLOCK 64
 #define TILE_NODES 216
 
+// A block's P2G tile (its 4^3 nodes and the +1 halo, 6^3 nodes) is stored region
+// by region, so that what the grid update reads from each neighbour's tile is
+// contiguous: region r = (x >= 4) | (y >= 4) << 1 | (z >= 4) << 2 of extent 2
+// along the axes where it lies in the halo and 4 along the others.
+__device__ __host__ inline uint32_t tile_region_base(uint32_t r) {
+    // sizes 64, 32, 32, 16, 32, 16, 16, 8
+    return r == 0u ? 0u : r == 1u ? 64u : r == 2u ? 96u : r == 3u ? 128u : r == 4u ? 144u : r == 5u ? 176u
+                                                                                             : r == 6u ? 192u : 208u;
+}
+
+__device__ __host__ inline uint32_t tile_index(uint32_t x, uint32_t y, uint32_t z) {
+    const uint32_t ox = x >= 4u, oy = y >= 4u, oz = z >= 4u;
+    const uint32_t ex = ox ? 2u : 4u, ey = oy ? 2u : 4u;
+    return tile_region_base(ox | (oy << 1) | (oz << 2)) + (x - 4u * ox) + ex * ((y - 4u * oy) + ey * (z - 4u * oz));
+}
+
 // Counter slots of one substep's counter set (two sets alternate with the hash
 // tables so a substep can prepare the next one's).
 #define CTR_NUM_BLOCKS 0
 #define CTR_SORTED_ALLOC 1
 #define CTR_ERRORS 2
+#define CTR_MOVERS 3     // entries of the mover list destined for this table
+#define CTR_OVERFLOW 4   // movers beyond a block's MOVER_CAP list slots
 #define ERR_HASH_FULL 1u
 #define ERR_LOOKUP 2u
 #define ERR_KEY_RANGE 4u
+
+// Particle storage between substeps: every block's particles are contiguous,
+// those that stay in the block first, those that leave it last. A particle that
+// leaves (a "mover") is listed with its new block key by the substep that moved
+// it; the next substep's sort places the movers of each block in MOVER_CAP list
+// slots per block (and any beyond them in a shared overflow list).
+#define MOVER_CAP 64
 
 // Neighbour table per block (16 words): [0..6] the +1 neighbours in the
 // reference's associated-block order (grid.slang:371-379, offsets 1..7),
@@ -150,13 +179,13 @@ __device__ inline void assoc_block_cell(float h, float px, float py, float pz, i
     *cell = lx + 4u * ly + 16u * lz;
 }
 
-// Bounded open-addressing lookup (find_block_header_id, grid.slang:176-198).
-__device__ inline uint32_t hash_find(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
-                                     uint32_t cap, uint32_t key) {
+// Bounded open-addressing lookup (find_block_header_id, grid.slang:176-198);
+// returns the key's slot, which is the block's header id.
+__device__ inline uint32_t hash_find(const uint32_t *__restrict__ keys, uint32_t cap, uint32_t key) {
     uint32_t slot = hash_key(key) & (cap - 1u);
     for (uint32_t k = 0; k < cap; k++) {
         uint32_t st = keys[slot];
-        if (st == key) return vals[slot];
+        if (st == key) return slot;
         if (st == SLOSH_NONE) return SLOSH_NONE;
         slot = (slot + 1u) & (cap - 1u);
     }
@@ -184,19 +213,36 @@ inline void launch_kernel(const PassEvents *ev, void (*kernel)(KArgs...), dim3 g
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 __device__ inline uint64_t lanemask_lt() { return __lanemask_lt(); }
 
+// Per-block counters a new block must start from zero, cleared by the thread
+// whose insertion created the block (null pointers are skipped): its mover
+// count, and for the full sort its particle count and 64 cell counts.
+struct BlockInit {
+    uint32_t *mcount, *blk_cnt, *cell_cnt;
+};
+
 // Open-addressing insertion with a strong CAS (insertion_index, grid.slang:118-174).
 // A plain load first: most insertions find their key already present (many
 // runs of a wavefront, and many wavefronts, touch the same block), which then
 // costs no atomic. A stale NONE only leads to the CAS, which sees the truth.
 __device__ inline void hash_insert(uint32_t *__restrict__ keys, uint32_t cap, uint32_t key,
-                                   uint32_t *__restrict__ counters) {
+                                   uint32_t *__restrict__ counters, const BlockInit &init) {
     uint32_t slot = hash_key(key) & (cap - 1u);
     for (uint32_t k = 0; k < cap; k++) {
         const uint32_t cur = keys[slot];
         if (cur == key) return;
         if (cur == SLOSH_NONE) {
             const uint32_t old = atomicCAS(&keys[slot], SLOSH_NONE, key);
-            if (old == SLOSH_NONE || old == key) return;
+            if (old == SLOSH_NONE) {
+                if (init.mcount) init.mcount[slot] = 0u;
+                if (init.blk_cnt) init.blk_cnt[slot] = 0u;
+                if (init.cell_cnt) {
+                    uint4 *cc = reinterpret_cast<uint4 *>(init.cell_cnt + (size_t)slot * NODES_PER_BLOCK);
+#pragma unroll
+                    for (int j = 0; j < 16; j++) cc[j] = make_uint4(0u, 0u, 0u, 0u);
+                }
+                return;
+            }
+            if (old == key) return;
         }
         slot = (slot + 1u) & (cap - 1u);
     }
@@ -214,7 +260,7 @@ __device__ constexpr int BLOCK_ASSOC[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {
 // (head, offset) pairs, eight heads per round. Every lane of the wavefront must
 // call it.
 __device__ inline void insert_blocks(uint32_t *__restrict__ keys, uint32_t cap, uint32_t batched, uint32_t key,
-                                     uint64_t heads, uint32_t *__restrict__ counters) {
+                                     uint64_t heads, uint32_t *__restrict__ counters, const BlockInit &init) {
     const uint32_t lane = __lane_id();
     const uint32_t r = lane >> 3, o = lane & 7u;
     while (heads) {
@@ -228,7 +274,7 @@ __device__ inline void insert_blocks(uint32_t *__restrict__ keys, uint32_t cap, 
             unpack_key(batched, hk, &x, &y, &z, &scene);
             hash_insert(keys, cap,
                         pack_key(batched, scene, x + BLOCK_ASSOC[o][0], y + BLOCK_ASSOC[o][1], z + BLOCK_ASSOC[o][2]),
-                        counters);
+                        counters, init);
         }
 #pragma unroll
         for (int k = 0; k < 8; k++) heads &= heads ? heads - 1ull : 0ull;

@@ -13,7 +13,7 @@
 //   k_scene_gather     writes one scene's particles, in id order, to a staging
 //                      buffer set
 #include "../../include/slosh_hip.h"
-#include "common.h"
+#include "launch.h"
 
 namespace {
 
@@ -222,16 +222,6 @@ hipError_t scan_u32(uint32_t *data, uint32_t n, uint32_t *tile_sums, uint32_t *t
 
 uint32_t scan_tiles(uint32_t n) { return div_up_u(n > 0 ? n : 1u, SCAN_TILE); }
 
-struct CompactArgs {
-    // current and destination particle sets (slot order)
-    float4 *pos[2], *vm[2];
-    float *C[2], *F[2], *vgd[2], *dps[2], *fdt[2];
-    uint32_t *ids[2];
-    // by-id static records, current and destination
-    float4 *prop[2], *model[2], *dp0[2], *dp1[2];
-    uint32_t *keep_slot, *keep_id, *keep_id2, *tile_sums, *total;
-};
-
 hipError_t launch_compact(const CompactArgs &a, uint32_t n, uint32_t pcap, uint32_t mode, const uint8_t *dead,
                           uint32_t lo, uint32_t hi, hipStream_t st) {
     if (n == 0) return hipSuccess;
@@ -252,16 +242,6 @@ hipError_t launch_compact(const CompactArgs &a, uint32_t n, uint32_t pcap, uint3
                        (const uint32_t *)a.keep_id);
     return hipGetLastError();
 }
-
-struct GatherArgs {
-    float4 *pos, *vm;
-    float *C, *F, *vgd, *dps, *fdt;
-    uint32_t *ids;
-    // staging (count entries per component)
-    float4 *s_pos, *s_vm;
-    float *s_C, *s_F, *s_vgd, *s_dps, *s_fdt;
-    uint32_t *rank_id, *tile_sums, *total;
-};
 
 hipError_t launch_scene_gather(const GatherArgs &a, uint32_t n, uint32_t pcap, uint32_t scene, uint32_t count,
                                hipStream_t st) {

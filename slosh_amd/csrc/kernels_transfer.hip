@@ -20,19 +20,13 @@
 //   k_g2p / k_particle_update  the same two stages as separate in-place
 //                    kernels, used when a hook replaces or observes G2P.
 //   k_rigid_update   rigid_impulses.slang:70-121.
-#include "common.h"
+#include "launch.h"
 #include "../../include/slosh_hip.h"
 
 namespace {
 
 constexpr int ASSOC[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1},
                              {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
-
-// NBH_SHIFTS order of kernel.slang:120-149 (summation order of G2P).
-constexpr uint8_t NBH_C[27][3] = {
-    {2, 2, 2}, {2, 0, 2}, {2, 1, 2}, {0, 2, 2}, {0, 0, 2}, {0, 1, 2}, {1, 2, 2}, {1, 0, 2}, {1, 1, 2},
-    {2, 2, 0}, {2, 0, 0}, {2, 1, 0}, {0, 2, 0}, {0, 0, 0}, {0, 1, 0}, {1, 2, 0}, {1, 0, 0}, {1, 1, 0},
-    {2, 2, 1}, {2, 0, 1}, {2, 1, 1}, {0, 2, 1}, {0, 0, 1}, {0, 1, 1}, {1, 2, 1}, {1, 0, 1}, {1, 1, 1}};
 
 // ---------------------------------------------------------------------------
 // Quadratic B-spline (kernel.slang:160-174).
@@ -1140,24 +1134,90 @@ __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 
 }
 
 // ---------------------------------------------------------------------------
+// A block's particles ("members").
+
+// In the steady state a block's members are the particles of its range in the
+// previous substep's storage that are still in the block ("candidates": the
+// range is read whole and filtered by block; prev_stay of them pass) and the
+// movers listed for it (MOVER_CAP list slots per block, the rest in the
+// overflow list). After a full sort they are the sorted list
+// perm[blk_start, blk_start + blk_cnt).
+struct Members {
+    uint32_t fast;
+    const uint32_t *blist;                   // block list (header ids = table slots)
+    const uint32_t *nbr;                     // neighbour tables (NBR_PREV: id in the previous table)
+    const uint4 *bin;                        // per block: range (start, count) and stayers of its previous-table block
+    const uint32_t *mcount, *mlist;          // movers received per block, the first MOVER_CAP of them
+    const uint32_t *ovf_slot, *ovf_hid;      // the others
+    const uint32_t *perm, *blk_start, *blk_cnt;  // the full sort's result
+    const uint32_t *counters;                // CTR_NUM_BLOCKS, CTR_OVERFLOW of this substep
+};
+
+struct BlockRange {
+    uint32_t cand_lo, cand_n;  // candidates: storage slots [cand_lo, cand_lo + cand_n)
+    const uint32_t *list;      // then the members list[0, list_n)
+    uint32_t list_n;
+    uint32_t total;            // members, overflow movers included
+    bool overflow;             // some of the block's movers are in the overflow list
+};
+
+__device__ inline BlockRange block_range(const Members &M, uint32_t hid) {
+    BlockRange r;
+    if (M.fast) {
+        const uint4 in = M.bin[hid];
+        const uint32_t mc = M.mcount[hid];
+        r.cand_lo = in.x;
+        r.cand_n = in.y;
+        r.list = M.mlist + (size_t)hid * MOVER_CAP;
+        r.list_n = mc < MOVER_CAP ? mc : MOVER_CAP;
+        r.total = in.z + mc;
+        r.overflow = mc > MOVER_CAP;
+    } else {
+        r.cand_lo = 0u;
+        r.cand_n = 0u;
+        r.list = M.perm + M.blk_start[hid];
+        r.list_n = M.blk_cnt[hid];
+        r.total = r.list_n;
+        r.overflow = false;
+    }
+    return r;
+}
+
+// Storage slot of entry m of a block's sequence (candidates, then list members).
+__device__ inline uint32_t entry_slot(const BlockRange &r, uint32_t m) {
+    return m < r.cand_n ? r.cand_lo + m : r.list[m - r.cand_n];
+}
+
+// Block key of the particle at x (w = scene << 1 | enabled bits): the key the
+// sort files it under (a block outside the key space aliases another block of
+// its key space, and is filed, counted and processed under that key).
+__device__ inline uint32_t key_at(float h, uint32_t batched, float4 x) {
+    int bx, by, bz;
+    uint32_t cell;
+    assoc_block_cell(h, x.x, x.y, x.z, &bx, &by, &bz, &cell);
+    return pack_key(batched, __float_as_uint(x.w) >> 1, bx, by, bz);
+}
+
+// ---------------------------------------------------------------------------
 // Kernels.
 
 // P2G for one block per workgroup of 3 waves.
-// Phase 1 (per chunk of P2G_CHUNK sorted particles of the block): each thread
-// loads one particle with coalesced reads (particle storage follows the block
-// order of the previous substep) and stages in LDS what every node of its
-// stencil shares: mass, the APIC momentum at the stencil's first node
-// A0 = m v + C^T (x_0 - x_p), the particle's offset f from that node in cells
-// (the kernel weights follow from it in a few operations, cheaper than
+// Phase 1 (per chunk of up to P2G_CHUNK members of the block): each thread
+// loads one particle (stayers are contiguous in storage) and stages in LDS what
+// every node of its stencil shares: mass, the APIC momentum at the stencil's
+// first node A0 = m v + C^T (x_0 - x_p), the particle's offset f from that node
+// in cells (the kernel weights follow from it in a few operations, cheaper than
 // staging them: LDS per particle bounds how many workgroups share a CU), and C.
+// The chunk is grouped by associated cell in LDS (a rank per particle from an
+// LDS counter per cell, a scan of the 64 counts, a list of staging indices).
 // Phase 2: lane = associated cell of the block, wave = z offset kz of the
 // stencil; each lane accumulates, over the particles of its cell in the chunk,
 // the contributions w_n (A0 + C^T (n h)) to the 9 nodes n = (kx, ky, kz) in 36
 // registers.
 // Phase 3: the waves merge their registers into the 6x6x6 LDS tile one after
 // the other, in 9 rounds in which the 64 lanes of a wave touch 64 distinct nodes
-// (plain read-modify-write; no LDS or global atomics). The summation order is
-// fixed by the particle order of each cell.
+// (plain read-modify-write; no LDS or global atomics on floats). Each cell's
+// summation order follows the order its particles were grouped in.
 // Cells hold ~8 particles, so lanes read particles ~8 apart: pad one word every
 // 8 to spread them over the LDS banks.
 __device__ inline uint32_t p2g_slot(uint32_t q) { return q + (q >> 3); }
@@ -1166,142 +1226,275 @@ __device__ inline uint32_t p2g_slot(uint32_t q) { return q + (q >> 3); }
 // fused G2P / particle-update kernel (replaces k_reset_table + k_touch_primary
 // of the next substep).
 struct NextTable {
-    uint32_t *keys, *vals, *counters;
+    uint32_t *keys, *counters;
     uint32_t prime;
 };
+
+// Block totals and starts of a steady-state substep, in block-list order, by
+// one workgroup of 192 threads: total = stayers of the block's previous range +
+// movers received. Runs beside the per-block work of the same launch (nothing
+// in it reads them).
+__device__ void p2g_block_scan(const Members &M, uint32_t *__restrict__ blk_start, uint32_t *__restrict__ blk_cnt,
+                               uint32_t *wsum) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t B = M.counters[CTR_NUM_BLOCKS];
+    // 24 entries per thread: up to 4608 blocks in one pass of two dependent loads.
+    constexpr uint32_t PER = 24u, TILE = 192u * PER;
+    uint32_t carry = 0u;
+    for (uint32_t base = 0; base < B; base += TILE) {
+        uint32_t hid[PER], c[PER];
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) {
+            const uint32_t k = base + tid * PER + j;
+            hid[j] = k < B ? M.blist[k] : SLOSH_NONE;
+        }
+        uint32_t t = 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) {
+            c[j] = hid[j] != SLOSH_NONE ? M.bin[hid[j]].z + M.mcount[hid[j]] : 0u;
+            t += c[j];
+        }
+        uint32_t x = t;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d);
+            if ((int)lane >= d) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        const uint32_t w0 = wsum[0], w1 = wsum[1], w2 = wsum[2];
+        uint32_t e = carry + (wave > 0 ? w0 : 0u) + (wave > 1 ? w1 : 0u) + x - t;
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) {
+            if (hid[j] != SLOSH_NONE) {
+                blk_start[hid[j]] = e;
+                blk_cnt[hid[j]] = c[j];
+            }
+            e += c[j];
+        }
+        carry += w0 + w1 + w2;
+        __syncthreads();  // wsum is reused by the next tile
+    }
+}
+
+// Global index of LDS tile node j (x + 6 y + 36 z) in the region layout.
+__device__ inline uint32_t tile_global_of_lds(uint32_t j) {
+    const uint32_t x = j % 6u, y = (j / 6u) % 6u, z = j / 36u;
+    return tile_index(x, y, z);
+}
+
+template <int P2G_CHUNK>
+struct P2GShared {
+    static constexpr int LDS = P2G_CHUNK + P2G_CHUNK / 8;
+    float4 tile[TILE_NODES];
+    float sp[16][LDS];  // m, A0[3], f[3], C[9], grouped by associated cell
+    uint32_t lcnt[64], lstart[64];
+    uint32_t extra[192];  // this block's overflow movers of one round
+    uint32_t wsum[4];
+};
+
+// Phase 1 and 2 for `kn` entries of the block's sequence; entry t's storage slot
+// is slot_of(t). An entry is a member if its associated block is this block
+// (candidates that moved to another block are skipped; every other entry is one
+// by construction). Every thread of the workgroup calls it. `tot` (wave 0, lane
+// = cell) accumulates the block's cell counts.
+template <int P2G_CHUNK, typename SlotOf>
+__device__ inline void p2g_chunk(const StepParams &P, P2GShared<P2G_CHUNK> &S, uint32_t kn, SlotOf slot_of, const float4 *__restrict__ pos, const float4 *__restrict__ vm,
+                                 const float *__restrict__ C, const float *__restrict__ fdt, uint32_t pcap,
+                                 uint32_t block_key, float (&acc)[9][4], uint32_t &tot) {
+    constexpr int SP_M = 0, SP_A = 1, SP_F = 4, SP_C = 7;
+    constexpr int R = (P2G_CHUNK + 191) / 192;
+    const uint32_t kz = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const float h = P.h;
+    const float kzh = (float)kz * h;
+    __syncthreads();  // the previous chunk's readers are done; the cell counters are clear
+    // All of this thread's loads are issued before any is consumed.
+    uint32_t pp[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t t = threadIdx.x + 192u * r;
+        pp[r] = t < kn ? slot_of(t) : SLOSH_NONE;
+    }
+    float4 ps[R], vv[R];
+    float cc[R][9];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (pp[r] != SLOSH_NONE) {
+            ps[r] = pos[pp[r]];
+            vv[r] = vm[pp[r]];
+#pragma unroll
+            for (int k = 0; k < 9; k++) cc[r][k] = C[(size_t)k * pcap + pp[r]];
+        }
+    }
+    // Per member: the associated cell, a rank within it, and the staged terms.
+    uint32_t cl[R], rk[R];
+    float a0[R][3], fo[R][3];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (pp[r] == SLOSH_NONE) continue;
+        float mom[3] = {vv[r].x * vv[r].w, vv[r].y * vv[r].w, vv[r].z * vv[r].w};
+        if (P.has_force) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) mom[i] += fdt[(size_t)i * pcap + pp[r]];
+        }
+        // The associated cell: node 0 of the stencil sits at d0 = x_0 - x_p; its
+        // block (floor of the cell / 4) and its place in that block.
+        const float xp[3] = {ps[r].x, ps[r].y, ps[r].z};
+        float d0[3];
+        int blk[3], lc[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float ac = rintf(div_h(xp[i], h)) - 1.0f;
+            d0[i] = ac * h - xp[i];
+            fo[r][i] = div_h(-d0[i], h);
+            const float fb = floorf(ac / 4.0f);
+            blk[i] = (int)fb;
+            lc[i] = (int)(ac - fb * 4.0f);
+        }
+        // A candidate that moved to another block is that block's member.
+        if (pack_key(P.batched, __float_as_uint(ps[r].w) >> 1, blk[0], blk[1], blk[2]) != block_key) {
+            pp[r] = SLOSH_NONE;
+            continue;
+        }
+        const float *c = cc[r];
+#pragma unroll
+        for (int i = 0; i < 3; i++) a0[r][i] = mom[i] + (d0[0] * c[i] + d0[1] * c[3 + i] + d0[2] * c[6 + i]);
+        cl[r] = (uint32_t)(lc[0] + 4 * lc[1] + 16 * lc[2]);
+        rk[r] = atomicAdd(&S.lcnt[cl[r]], 1u);
+    }
+    __syncthreads();
+    // Every wave scans the 64 counts (lane = cell) and writes the same starts.
+    const uint32_t cnt = S.lcnt[lane];
+    uint32_t x = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if ((int)lane >= d) x += y;
+    }
+    const uint32_t cs = x - cnt;
+    S.lstart[lane] = cs;
+    if (kz == 0) tot += cnt;
+    __syncthreads();
+    // Staging in cell order: a cell's members are consecutive.
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (pp[r] == SLOSH_NONE) continue;
+        const uint32_t q = p2g_slot(S.lstart[cl[r]] + rk[r]);
+        S.sp[SP_M][q] = vv[r].w;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            S.sp[SP_A + i][q] = a0[r][i];
+            S.sp[SP_F + i][q] = fo[r][i];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) S.sp[SP_C + k][q] = cc[r][k];
+    }
+    __syncthreads();
+    for (uint32_t j = cs; j < cs + cnt; j++) {
+        const uint32_t q = p2g_slot(j);
+        const float m = S.sp[SP_M][q];
+        const float wz = k_eval(S.sp[SP_F + 2][q] - (float)kz);
+        float c[9], wx[3], wy[3], wyz[3], az[3];
+        k_eval_all(S.sp[SP_F][q], wx);
+        k_eval_all(S.sp[SP_F + 1][q], wy);
+#pragma unroll
+        for (int k = 0; k < 9; k++) c[k] = S.sp[SP_C + k][q];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            wyz[i] = wy[i] * wz;
+            // A0 + C^T (0, 0, kz h): shared by the 9 nodes of the slab.
+            az[i] = S.sp[SP_A + i][q] + kzh * c[6 + i];
+        }
+#pragma unroll
+        for (int ky = 0; ky < 3; ky++) {
+            float ay[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) ay[i] = ky == 0 ? az[i] : az[i] + (float)ky * h * c[3 + i];
+#pragma unroll
+            for (int kx = 0; kx < 3; kx++) {
+                const int k = kx + 3 * ky;
+                const float wgt = wx[kx] * wyz[ky];
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const float a = kx == 0 ? ay[i] : ay[i] + (float)kx * h * c[i];
+                    acc[k][i] += a * wgt;
+                }
+                acc[k][3] += m * wgt;
+            }
+        }
+    }
+    __syncthreads();
+    if (kz == 0) S.lcnt[lane] = 0u;
+}
 
 template <int P2G_CHUNK>
 __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const float4 *__restrict__ pos,
                                              const float4 *__restrict__ vm, const float *__restrict__ C,
-                                             const float *__restrict__ fdt, uint32_t pcap,
-                                             const uint32_t *__restrict__ perm,
-                                             const uint32_t *__restrict__ cell_cnt,
-                                             const uint32_t *__restrict__ cell_start,
-                                             const uint32_t *__restrict__ blk_start,
-                                             const uint32_t *__restrict__ blk_cnt,
-                                             const uint32_t *__restrict__ hdr_key, float4 *__restrict__ tiles,
-                                             const uint32_t *__restrict__ counters) {
-    __shared__ float4 tile[TILE_NODES];
-    constexpr int P2G_LDS = P2G_CHUNK + P2G_CHUNK / 8;
-    constexpr int R = (P2G_CHUNK + 191) / 192;
-    // m, A0[3], f[3], C[9]
-    constexpr int SP_M = 0, SP_A = 1, SP_F = 4, SP_C = 7;
-    __shared__ float sp[16][P2G_LDS];
+                                             const float *__restrict__ fdt, uint32_t pcap, Members M,
+                                             const uint32_t *__restrict__ keys, float4 *__restrict__ tiles,
+                                             uint32_t *__restrict__ cell_cnt, uint32_t *__restrict__ blk_start_out,
+                                             uint32_t *__restrict__ blk_cnt_out) {
+    __shared__ P2GShared<P2G_CHUNK> S;
     if (nt.prime) {
-        for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < P.cap; s += gridDim.x * blockDim.x) {
+        for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < P.cap; s += gridDim.x * blockDim.x)
             nt.keys[s] = SLOSH_NONE;
-            nt.vals[s] = 0u;
-        }
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             nt.counters[CTR_NUM_BLOCKS] = 0u;
             nt.counters[CTR_SORTED_ALLOC] = 0u;
+            nt.counters[CTR_MOVERS] = 0u;
+            nt.counters[CTR_OVERFLOW] = 0u;
         }
     }
-    const uint32_t B = counters[CTR_NUM_BLOCKS];
+    // Steady state: workgroup 0 computes the block totals and starts.
+    const uint32_t first = M.fast ? 1u : 0u;
+    if (M.fast && blockIdx.x == 0) {
+        p2g_block_scan(M, blk_start_out, blk_cnt_out, S.wsum);
+        return;
+    }
+    const uint32_t B = M.counters[CTR_NUM_BLOCKS];
+    const uint32_t n_ovf = M.counters[CTR_OVERFLOW];
     const uint32_t kz = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
     const uint32_t tbase = lx + 6 * ly + 36 * (lz + (int)kz);
-    const float h = P.h;
-    const float kzh = (float)kz * h;
-    for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
-        const uint32_t b0 = blk_start[hid], nb = blk_cnt[hid];
+    if (threadIdx.x < 64) S.lcnt[threadIdx.x] = 0u;
+    for (uint32_t k = blockIdx.x - first; k < B; k += gridDim.x - first) {
+        const uint32_t hid = M.blist[k];
+        const BlockRange br = block_range(M, hid);
         // A block without particles (a "+1" neighbour only) has an all-zero tile:
         // nothing is written and the grid update skips it.
-        if (nb == 0) continue;
-        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 192) tile[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        int bx, by, bz;
-        uint32_t scene;
-        unpack_key(P.batched, hdr_key[hid], &bx, &by, &bz, &scene);
-        const uint32_t cs = cell_start[(size_t)hid * NODES_PER_BLOCK + lane];
-        const uint32_t ce = cs + cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane];
+        if (br.total == 0) {
+            if (threadIdx.x < 64) cell_cnt[(size_t)hid * NODES_PER_BLOCK + threadIdx.x] = 0u;
+            continue;
+        }
+        __syncthreads();  // the previous block's tile has been written out
+        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 192) S.tile[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const uint32_t block_key = keys[hid];
         float acc[9][4];
 #pragma unroll
-        for (int k = 0; k < 9; k++) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.0f;
-        for (uint32_t chunk = 0; chunk < nb; chunk += P2G_CHUNK) {
-            __syncthreads();
-            // All of this thread's loads are issued before any is consumed.
-            uint32_t pp[R];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                const uint32_t t = threadIdx.x + 192u * r;
-                pp[r] = (t < (uint32_t)P2G_CHUNK && chunk + t < nb) ? perm[b0 + chunk + t] : SLOSH_NONE;
-            }
-            float4 ps[R], vv[R];
-            float cc[R][9];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                if (pp[r] != SLOSH_NONE) {
-                    ps[r] = pos[pp[r]];
-                    vv[r] = vm[pp[r]];
-#pragma unroll
-                    for (int k = 0; k < 9; k++) cc[r][k] = C[(size_t)k * pcap + pp[r]];
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                if (pp[r] == SLOSH_NONE) continue;
-                const uint32_t p = pp[r];
-                float mom[3] = {vv[r].x * vv[r].w, vv[r].y * vv[r].w, vv[r].z * vv[r].w};
-                if (P.has_force) {
-#pragma unroll
-                    for (int i = 0; i < 3; i++) mom[i] += fdt[(size_t)i * pcap + p];
-                }
-                // The associated cell (the block and cell the sort filed the
-                // particle under): node 0 of the stencil sits at d0 = x_0 - x_p.
-                const float xp[3] = {ps[r].x, ps[r].y, ps[r].z};
-                float d0[3], f[3];
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    const float ac = rintf(div_h(xp[i], h)) - 1.0f;
-                    d0[i] = ac * h - xp[i];
-                    f[i] = div_h(-d0[i], h);
-                }
-                const float *c = cc[r];
-                const uint32_t q = p2g_slot(threadIdx.x + 192u * r);
-                sp[SP_M][q] = vv[r].w;
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    sp[SP_A + i][q] = mom[i] + (d0[0] * c[i] + d0[1] * c[3 + i] + d0[2] * c[6 + i]);
-                    sp[SP_F + i][q] = f[i];
-                }
-#pragma unroll
-                for (int k = 0; k < 9; k++) sp[SP_C + k][q] = c[k];
-            }
-            __syncthreads();
-            const uint32_t j0 = cs > chunk ? cs : chunk;
-            const uint32_t j1 = ce < chunk + P2G_CHUNK ? ce : chunk + P2G_CHUNK;
-            for (uint32_t j = j0; j < j1; j++) {
-                const uint32_t q = p2g_slot(j - chunk);
-                const float m = sp[SP_M][q];
-                const float wz = k_eval(sp[SP_F + 2][q] - (float)kz);
-                float c[9], wx[3], wy[3], wyz[3], az[3];
-                k_eval_all(sp[SP_F][q], wx);
-                k_eval_all(sp[SP_F + 1][q], wy);
-#pragma unroll
-                for (int k = 0; k < 9; k++) c[k] = sp[SP_C + k][q];
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    wyz[i] = wy[i] * wz;
-                    // A0 + C^T (0, 0, kz h): shared by the 9 nodes of the slab.
-                    az[i] = sp[SP_A + i][q] + kzh * c[6 + i];
-                }
-#pragma unroll
-                for (int ky = 0; ky < 3; ky++) {
-                    float ay[3];
-#pragma unroll
-                    for (int i = 0; i < 3; i++) ay[i] = ky == 0 ? az[i] : az[i] + (float)ky * h * c[3 + i];
-#pragma unroll
-                    for (int kx = 0; kx < 3; kx++) {
-                        const int k = kx + 3 * ky;
-                        const float wgt = wx[kx] * wyz[ky];
-#pragma unroll
-                        for (int i = 0; i < 3; i++) {
-                            const float a = kx == 0 ? ay[i] : ay[i] + (float)kx * h * c[i];
-                            acc[k][i] += a * wgt;
-                        }
-                        acc[k][3] += m * wgt;
-                    }
-                }
+        for (int q = 0; q < 9; q++) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.0f;
+        uint32_t tot = 0u;
+        const uint32_t n_main = br.cand_n + br.list_n;
+        for (uint32_t chunk = 0; chunk < n_main; chunk += P2G_CHUNK) {
+            const uint32_t kn = n_main - chunk < (uint32_t)P2G_CHUNK ? n_main - chunk : (uint32_t)P2G_CHUNK;
+            p2g_chunk<P2G_CHUNK>(P, S, kn, [&](uint32_t t) { return entry_slot(br, chunk + t); }, pos, vm, C, fdt,
+                                 pcap, block_key, acc, tot);
+        }
+        if (br.overflow) {
+            // This block's movers beyond its list: picked out of the overflow list,
+            // one round of 192 entries at a time.
+            for (uint32_t ob = 0; ob < n_ovf; ob += 192u) {
+                const uint32_t i = ob + threadIdx.x;
+                const bool mine = i < n_ovf && M.ovf_hid[i] == hid;
+                const uint64_t mm = __ballot(mine);
+                __syncthreads();  // the previous round's readers of `extra` are done
+                if (lane == 0) S.wsum[kz] = (uint32_t)__popcll(mm);
+                __syncthreads();
+                const uint32_t before = (kz > 0 ? S.wsum[0] : 0u) + (kz > 1 ? S.wsum[1] : 0u);
+                const uint32_t kn = S.wsum[0] + S.wsum[1] + S.wsum[2];
+                if (mine) S.extra[before + (uint32_t)__popcll(mm & lanemask_lt())] = M.ovf_slot[i];
+                if (kn > 0)
+                    p2g_chunk<P2G_CHUNK>(P, S, kn, [&](uint32_t t) { return S.extra[t]; }, pos, vm, C, fdt, pcap,
+                                         block_key, acc, tot);
             }
         }
         __syncthreads();
@@ -1312,14 +1505,14 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
                 for (int ky = 0; ky < 3; ky++)
 #pragma unroll
                     for (int kx = 0; kx < 3; kx++) {
-                        const int k = kx + 3 * ky;
+                        const int q = kx + 3 * ky;
                         const uint32_t node = tbase + kx + 6 * ky;
-                        float4 t = tile[node];
-                        t.x += acc[k][0];
-                        t.y += acc[k][1];
-                        t.z += acc[k][2];
-                        t.w += acc[k][3];
-                        tile[node] = t;
+                        float4 t = S.tile[node];
+                        t.x += acc[q][0];
+                        t.y += acc[q][1];
+                        t.z += acc[q][2];
+                        t.w += acc[q][3];
+                        S.tile[node] = t;
                         // The next round's nodes overlap other lanes' nodes of this
                         // round: complete this round's LDS writes first.
                         __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
@@ -1327,8 +1520,9 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
             }
             __syncthreads();
         }
-        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 192) tiles[(size_t)hid * TILE_NODES + j] = tile[j];
-        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 192)
+            tiles[(size_t)hid * TILE_NODES + tile_global_of_lds(j)] = S.tile[j];
+        if (kz == 0) cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane] = tot;
     }
 }
 
@@ -1337,7 +1531,8 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
 template <bool MESH>
 __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 *__restrict__ tiles,
                                                      float4 *__restrict__ nodes, const uint32_t *__restrict__ nbr,
-                                                     const uint32_t *__restrict__ hdr_key, Colliders cs,
+                                                     const uint32_t *__restrict__ blist,
+                                                     const uint32_t *__restrict__ keys, Colliders cs,
                                                      const NodeColl *__restrict__ coll_prev, NodeColl *__restrict__ coll_cur,
                                                      const uint32_t *__restrict__ counters,
                                                      const uint32_t *__restrict__ blk_cnt, uint32_t mode,
@@ -1349,17 +1544,19 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const int x = lane & 3, y = (lane >> 2) & 3, z = lane >> 4;
     const uint32_t waves_per_grid = gridDim.x * 4;
-    for (uint32_t hid = blockIdx.x * 4 + wave; hid < B; hid += waves_per_grid) {
+    for (uint32_t k = blockIdx.x * 4 + wave; k < B; k += waves_per_grid) {
+        const uint32_t hid = blist[k];
         float4 acc;
         if (mode == 0) {
             // Tiles of blocks without particles were not written (they count as
             // zero). Each tile value is loaded together with its block's particle
             // count and then selected by it, so the tile loads wait only for the
             // neighbour ids, not for the counts (every tile lies inside the
-            // allocation, written or not).
+            // allocation, written or not). The part of a "-1" neighbour's tile
+            // that lands on this block is one contiguous region of that tile.
             float4 t[8];
             uint32_t c[8];
-            t[0] = tiles[(size_t)hid * TILE_NODES + x + 6 * y + 36 * z];
+            t[0] = tiles[(size_t)hid * TILE_NODES + x + 4 * y + 16 * z];
             c[0] = blk_cnt[hid];
 #pragma unroll
             for (int d = 1; d < 8; d++) {
@@ -1369,7 +1566,7 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
                 if ((ox == 0 || x <= 1) && (oy == 0 || y <= 1) && (oz == 0 || z <= 1)) {
                     const uint32_t src = nbr[(size_t)hid * NBR_STRIDE + NBR_MINUS + d - 1];
                     if (src != SLOSH_NONE) {
-                        t[d] = tiles[(size_t)src * TILE_NODES + (x + 4 * ox) + 6 * (y + 4 * oy) + 36 * (z + 4 * oz)];
+                        t[d] = tiles[(size_t)src * TILE_NODES + tile_index(x + 4 * ox, y + 4 * oy, z + 4 * oz)];
                         c[d] = blk_cnt[src];
                     }
                 }
@@ -1401,7 +1598,7 @@ __global__ __launch_bounds__(256) void k_grid_update(StepParams P, const float4 
         }
         int bx, by, bz;
         uint32_t scene;
-        unpack_key(P.batched, hdr_key[hid], &bx, &by, &bz, &scene);
+        unpack_key(P.batched, keys[hid], &bx, &by, &bz, &scene);
         const float h = P.h, dt = P.dt;
         float3 cell_pos = make_float3((float)(bx * 4 + x) * h, (float)(by * 4 + y) * h, (float)(bz * 4 + z) * h);
         // update_single_cell, grid_update.slang:154-163
@@ -1522,101 +1719,187 @@ __device__ inline void load_halo(float4 *__restrict__ sh, const float4 *__restri
     }
 }
 
+// What the fused G2P kernel prepares for the next substep: its table (keys
+// inserted, a new block's mover count cleared), the list of particles that
+// leave their block, and every block's stayer count.
+struct NextSort {
+    uint32_t *mcount;         // next table: movers per block (cleared at insertion)
+    uint32_t *mv_key, *mv_slot;  // movers: new block key, output slot
+    uint32_t *stay;           // this table: particles of each block that stay in it
+    uint32_t *errors;         // this substep's error word (ERR_KEY_RANGE)
+};
+
 // F64: the SVD-based steps refined to FP64 accuracy. The bound keeps that
 // variant at three waves per SIMD (168 VGPRs).
+//
+// Output order: a block's members take the output slots [start, start + total)
+// of the block in order of arrival of their wavefronts (one LDS atomic per
+// wavefront), so a block's particles stay together in storage. The particles
+// whose updated position leaves the block are listed, with their new block key,
+// for the next substep's sort; the others are counted (the block's stayers).
 template <uint32_t MODELS, int NT, bool F64>
 __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
-                                                    float *__restrict__ fdt, const uint32_t *__restrict__ perm,
+                                                    float *__restrict__ fdt, Members M,
                                                     const uint32_t *__restrict__ blk_start,
-                                                    const uint32_t *__restrict__ blk_cnt,
-                                                    const uint32_t *__restrict__ nbr, const float4 *__restrict__ nodes,
-                                                    const uint32_t *__restrict__ counters, NextTable nt, Bodies bodies,
-                                                    const uint32_t *__restrict__ hdr_key) {
+                                                    const float4 *__restrict__ nodes,
+                                                    const uint32_t *__restrict__ keys, NextTable nt, NextSort ns,
+                                                    Bodies bodies) {
     __shared__ float4 sh[TILE_NODES];
+    __shared__ uint32_t n_out, n_stay;
     // integrate_bodies: reads no particle or grid state (the impulses it applies
     // are only written by the host), so it runs alongside the particle update.
     if (blockIdx.x == 0 && threadIdx.x < bodies.count) integrate_body(P, bodies, threadIdx.x);
-    const uint32_t B = counters[CTR_NUM_BLOCKS];
-    for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
-        const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
-        if (cnt == 0) continue;
-        const uint32_t block_key = hdr_key[hid];
+    const uint32_t B = M.counters[CTR_NUM_BLOCKS];
+    const uint32_t n_ovf = M.counters[CTR_OVERFLOW];
+    const uint32_t lane = threadIdx.x & 63u;
+    BlockInit init;
+    init.mcount = ns.mcount;
+    init.blk_cnt = nullptr;
+    init.cell_cnt = nullptr;
+    for (uint32_t k = blockIdx.x; k < B; k += gridDim.x) {
+        const uint32_t hid = M.blist[k];
+        const BlockRange br = block_range(M, hid);
+        const uint32_t cnt = br.total;
+        if (cnt == 0) {
+            if (threadIdx.x == 0) ns.stay[hid] = 0u;
+            continue;
+        }
+        const uint32_t s0 = blk_start[hid];
+        const uint32_t block_key = keys[hid];
+        int bx, by, bz;
+        uint32_t scene;
+        unpack_key(P.batched, block_key, &bx, &by, &bz, &scene);
         bool stays = false;
-        load_halo(sh, nodes, nbr, hid);
+        load_halo(sh, nodes, M.nbr, hid);
+        if (threadIdx.x == 0) {
+            n_out = 0u;
+            n_stay = 0u;
+        }
         __syncthreads();
-        // Every lane of the workgroup runs the same number of iterations so the
-        // wave-level run detection of the next table's insertion sees all lanes.
-        for (uint32_t jb = 0; jb < cnt; jb += blockDim.x) {
-            const uint32_t j = jb + threadIdx.x;
-            const bool valid = j < cnt;
-            uint32_t next_key = 0u;
-            if (valid) {
-            const uint32_t slot = s0 + j;
-            const uint32_t p = perm[slot];
-            const float4 ps = in.pos[p];
-            const uint32_t wbits = __float_as_uint(ps.w);
-            uint32_t enabled = wbits & 1u;
-            float3 pos = make_float3(ps.x, ps.y, ps.z);
-            float3 v;
-            float aff[9], F[9];
-            float vgd;
-            g2p_particle(sh, P.h, P.inv_d, pos, enabled != 0u, &v, aff, &vgd);
-            // Uniform masses skip the 16-byte velocity / mass load (the velocity
-            // comes from the grid): only the update disables, and it zeroes the mass.
-            float mass = P.mass_uniform ? (enabled ? P.u_mass : 0.0f) : in.vm[p].w;
-#pragma unroll
-            for (int k = 0; k < 9; k++) F[k] = in.F[(size_t)k * pcap + p];
-            const uint32_t oid = in.ids[p];
-            const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
-            const float4 model = P.uniform ? P.u_model : st.model[oid];
+        // One entry of the block's sequence (a candidate is a member only if it is
+        // still in the block): G2P, update, output at the next free slot of the
+        // block's range, listed if it leaves the block, its blocks inserted into
+        // the next table. Every lane of the workgroup calls it the same number of
+        // times (the wave-level counting and run detection see all lanes).
+        auto particle = [&](bool valid, uint32_t p) {
+            // Every load of the particle is issued before its membership and
+            // output slot are known (a candidate that left the block is rare).
+            float4 ps = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float F[9];
             float dps[3] = {0.0f, 0.0f, 0.0f};
-            if (P.has_sand) {
+            uint32_t oid = 0u;
+            float vm_w = 0.0f;
+            if (valid) {
+                ps = in.pos[p];
 #pragma unroll
-                for (int k = 0; k < 3; k++) dps[k] = in.dps[(size_t)k * pcap + p];
-            }
-            update_particle<MODELS, F64>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
-            out.pos[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
-            out.vm[slot] = make_float4(v.x, v.y, v.z, mass);
+                for (int q = 0; q < 9; q++) F[q] = in.F[(size_t)q * pcap + p];
+                oid = in.ids[p];
+                // Uniform masses skip the 16-byte velocity / mass load (the velocity
+                // comes from the grid): only the update disables, and it zeroes the mass.
+                if (!P.mass_uniform) vm_w = in.vm[p].w;
+                if (P.has_sand) {
 #pragma unroll
-            for (int k = 0; k < 9; k++) {
-                out.C[(size_t)k * pcap + slot] = aff[k];
-                out.F[(size_t)k * pcap + slot] = F[k];
+                    for (int q = 0; q < 3; q++) dps[q] = in.dps[(size_t)q * pcap + p];
+                }
             }
-            out.vgd[slot] = vgd;
-            out.ids[slot] = oid;
-            if (P.has_sand) {
+            const bool member = valid && key_at(P.h, P.batched, ps) == block_key;
+            const uint64_t lt = lanemask_lt();
+            const uint64_t mm = __ballot(member);
+            uint32_t ob = 0u;
+            if (lane == 0 && mm) ob = atomicAdd(&n_out, (uint32_t)__popcll(mm));
+            const uint32_t j = __shfl(ob, 0) + (uint32_t)__popcll(mm & lt);
+            const uint32_t slot = s0 + j;
+            uint32_t next_key = 0u;
+            bool in_range = true;
+            if (member && j < cnt) {
+                const uint32_t wbits = __float_as_uint(ps.w);
+                uint32_t enabled = wbits & 1u;
+                float3 pos = make_float3(ps.x, ps.y, ps.z);
+                float3 v;
+                float aff[9];
+                float vgd;
+                g2p_particle(sh, P.h, P.inv_d, pos, enabled != 0u, &v, aff, &vgd);
+                float mass = P.mass_uniform ? (enabled ? P.u_mass : 0.0f) : vm_w;
+                const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
+                const float4 model = P.uniform ? P.u_model : st.model[oid];
+                update_particle<MODELS, F64>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
+                out.pos[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
+                out.vm[slot] = make_float4(v.x, v.y, v.z, mass);
 #pragma unroll
-                for (int k = 0; k < 3; k++) out.dps[(size_t)k * pcap + slot] = dps[k];
-            }
-            if (P.has_force) {
-                fdt[slot] = 0.0f;
-                fdt[(size_t)pcap + slot] = 0.0f;
-                fdt[2 * (size_t)pcap + slot] = 0.0f;
-            }
-            if (nt.prime) {
+                for (int q = 0; q < 9; q++) {
+                    out.C[(size_t)q * pcap + slot] = aff[q];
+                    out.F[(size_t)q * pcap + slot] = F[q];
+                }
+                out.vgd[slot] = vgd;
+                out.ids[slot] = oid;
+                if (P.has_sand) {
+#pragma unroll
+                    for (int q = 0; q < 3; q++) out.dps[(size_t)q * pcap + slot] = dps[q];
+                }
+                if (P.has_force) {
+                    fdt[slot] = 0.0f;
+                    fdt[(size_t)pcap + slot] = 0.0f;
+                    fdt[2 * (size_t)pcap + slot] = 0.0f;
+                }
+                int nx, ny, nz;
                 uint32_t cell;
-                next_key = particle_key(make_float4(pos.x, pos.y, pos.z, __uint_as_float(wbits)), P.h, P.batched, &cell);
+                assoc_block_cell(P.h, pos.x, pos.y, pos.z, &nx, &ny, &nz, &cell);
+                next_key = pack_key(P.batched, wbits >> 1, nx, ny, nz);
+                in_range = block_in_key_range(P.batched, nx, ny, nz);
             }
-            }
+            // A block outside the key space would alias another one: reported,
+            // once per wavefront.
+            const uint64_t bad = __ballot(!in_range);
+            if (bad && lane == (uint32_t)(__ffsll((unsigned long long)bad) - 1)) atomicOr(ns.errors, ERR_KEY_RANGE);
+            const bool mover = member && j < cnt && next_key != block_key;
+            const bool stay = member && j < cnt && !mover;
+            stays |= stay;
+            const uint64_t sm = __ballot(stay);
+            if (lane == 0 && sm) atomicAdd(&n_stay, (uint32_t)__popcll(sm));
             if (nt.prime) {
+                // The movers, listed for the next substep's sort.
+                const uint64_t vm_ = __ballot(mover);
+                if (vm_) {
+                    uint32_t lb = 0u;
+                    if (lane == 0) lb = atomicAdd(&nt.counters[CTR_MOVERS], (uint32_t)__popcll(vm_));
+                    lb = __shfl(lb, 0);
+                    if (mover) {
+                        const uint32_t e = lb + (uint32_t)__popcll(vm_ & lt);
+                        ns.mv_key[e] = next_key;
+                        ns.mv_slot[e] = slot;
+                    }
+                }
                 // touch_primary_blocks and touch_neighbor_blocks of the next
                 // substep (sort.slang:50-92). Particles that stay in this block
                 // only flag it (its 8 blocks are inserted once below); the others
                 // insert theirs, one round per run of equal keys.
-                const bool mover = valid && next_key != block_key;
-                stays |= valid && !mover;
-                const uint64_t heads = run_heads(next_key, mover) & __ballot(mover);
-                insert_blocks(nt.keys, P.cap, P.batched, next_key, heads, nt.counters);
+                const uint64_t heads = run_heads(next_key, mover) & vm_;
+                insert_blocks(nt.keys, P.cap, P.batched, next_key, heads, nt.counters, init);
             }
+        };
+        // The block's entries, then (only if some of its movers overflowed their
+        // list) the overflow list, whose entries of other blocks are skipped.
+        const uint32_t n_main = br.cand_n + br.list_n;
+        const uint32_t n_all = n_main + (br.overflow ? n_ovf : 0u);
+        for (uint32_t jb = 0; jb < n_all; jb += blockDim.x) {
+            const uint32_t m = jb + threadIdx.x;
+            bool valid = m < n_all;
+            uint32_t p = 0u;
+            if (m < n_main) {
+                p = entry_slot(br, m);
+            } else if (valid) {
+                valid = M.ovf_hid[m - n_main] == hid;
+                p = M.ovf_slot[m - n_main];
+            }
+            particle(valid, p);
         }
-        if (nt.prime && __syncthreads_or(stays) && threadIdx.x < 8) {
-            int x, y, z;
-            uint32_t scene;
-            unpack_key(P.batched, block_key, &x, &y, &z, &scene);
+        const bool any_stays = __syncthreads_or(stays);
+        if (threadIdx.x == 0) ns.stay[hid] = n_stay;
+        if (nt.prime && any_stays && threadIdx.x < 8) {
             hash_insert(nt.keys, P.cap,
-                        pack_key(P.batched, scene, x + BLOCK_ASSOC[threadIdx.x][0], y + BLOCK_ASSOC[threadIdx.x][1],
-                                 z + BLOCK_ASSOC[threadIdx.x][2]),
-                        nt.counters);
+                        pack_key(P.batched, scene, bx + BLOCK_ASSOC[threadIdx.x][0], by + BLOCK_ASSOC[threadIdx.x][1],
+                                 bz + BLOCK_ASSOC[threadIdx.x][2]),
+                        nt.counters, init);
         }
         __syncthreads();
     }
@@ -1670,12 +1953,14 @@ __global__ __launch_bounds__(256) void k_timestep_bound(StepParams P, PBuf buf, 
 
 // G2P only, in place (hook path): writes velocity, velocity gradient, vel_grad_det.
 __global__ __launch_bounds__(256) void k_g2p(StepParams P, PBuf buf, uint32_t pcap, const uint32_t *__restrict__ perm,
+                                             const uint32_t *__restrict__ blist,
                                              const uint32_t *__restrict__ blk_start,
                                              const uint32_t *__restrict__ blk_cnt, const uint32_t *__restrict__ nbr,
                                              const float4 *__restrict__ nodes, const uint32_t *__restrict__ counters) {
     __shared__ float4 sh[TILE_NODES];
     const uint32_t B = counters[CTR_NUM_BLOCKS];
-    for (uint32_t hid = blockIdx.x; hid < B; hid += gridDim.x) {
+    for (uint32_t k = blockIdx.x; k < B; k += gridDim.x) {
+        const uint32_t hid = blist[k];
         const uint32_t s0 = blk_start[hid], cnt = blk_cnt[hid];
         if (cnt == 0) continue;
         load_halo(sh, nodes, nbr, hid);
@@ -1749,43 +2034,6 @@ __global__ __launch_bounds__(256, F64 ? 3 : 1) void k_particle_update(StepParams
 // ---------------------------------------------------------------------------
 // Host launchers (declarations in slosh_api.hip).
 
-struct TransferBuffers {
-    const uint32_t *perm, *cell_cnt, *cell_start, *blk_start, *blk_cnt, *hdr_key, *nbr, *counters;
-    float4 *tiles, *nodes;
-    void *coll_prev, *coll_cur;
-    float *fdt;
-};
-
-struct ParticleBuffers {
-    float4 *pos, *vm;
-    float *C, *F, *vgd, *dps;
-    uint32_t *ids;
-};
-
-struct StaticBuffers {
-    const float4 *prop, *model, *dp0, *dp1;
-};
-
-struct ColliderBuffers {
-    const float4 *shapes, *poses;
-    const uint2 *mats;
-    uint32_t count;
-    const float4 *mesh_vtx;    // trimesh collision vertices (x, y, z, pad)
-    const uint32_t *mesh_idx;  // trimesh collision indices
-};
-
-struct BodyBuffers {
-    float4 *poses, *vels;
-    const float4 *mprops, *local_com;
-    int *impulses;
-    uint32_t count;
-};
-
-struct NextTableBuffers {
-    uint32_t *keys, *vals, *counters;
-    uint32_t prime;
-};
-
 static inline uint32_t div_up_t(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
 static inline uint32_t min_t(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
@@ -1807,20 +2055,37 @@ static uint32_t block_grid(const StepParams &P, uint32_t per_wg) {
     return min_t(div_up_t(P.cap, per_wg), P.block_wgs ? P.block_wgs : 8192u);
 }
 
+static Members members_of(const TransferBuffers &t) {
+    Members M;
+    M.fast = t.fast;
+    M.blist = t.blist;
+    M.nbr = t.nbr;
+    M.bin = t.bin;
+    M.mcount = t.mcount;
+    M.mlist = t.mlist;
+    M.ovf_slot = t.ovf_slot;
+    M.ovf_hid = t.ovf_hid;
+    M.perm = t.perm;
+    M.blk_start = t.blk_start;
+    M.blk_cnt = t.blk_cnt;
+    M.counters = t.counters;
+    return M;
+}
+
 hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t pcap, const TransferBuffers &t,
                       const NextTableBuffers &next, const PassEvents *ev, hipStream_t stream) {
     NextTable nt;
     nt.keys = next.keys;
-    nt.vals = next.vals;
     nt.counters = next.counters;
     nt.prime = next.prime;
     auto kern = P.p2g_chunk == 192u ? k_p2g<192>
                 : P.p2g_chunk == 256u ? k_p2g<256>
                 : P.p2g_chunk == 384u ? k_p2g<384>
                                       : k_p2g<512>;
-    launch_kernel(ev, kern, dim3(block_grid(P, 1)), dim3(192), 0, stream, P, nt, (const float4 *)pb.pos,
-                  (const float4 *)pb.vm, (const float *)pb.C, (const float *)t.fdt, pcap, t.perm, t.cell_cnt,
-                  t.cell_start, t.blk_start, t.blk_cnt, t.hdr_key, t.tiles, t.counters);
+    // Steady state: one more workgroup (the first) computes the block starts.
+    launch_kernel(ev, kern, dim3(block_grid(P, 1) + (t.fast ? 1u : 0u)), dim3(192), 0, stream, P, nt,
+                  (const float4 *)pb.pos, (const float4 *)pb.vm, (const float *)pb.C, (const float *)t.fdt, pcap,
+                  members_of(t), t.keys, t.tiles, t.cell_cnt, t.blk_start, t.blk_cnt);
     return hipGetLastError();
 }
 
@@ -1837,8 +2102,8 @@ hipError_t launch_grid_update(const StepParams &P, const TransferBuffers &t, con
     // scenes without meshes keep the smaller kernel.
     launch_kernel(ev, c.mesh_vtx ? k_grid_update<true> : k_grid_update<false>, dim3(block_grid(P, 4)), dim3(256), 0,
                   stream, P,
-                  (const float4 *)t.tiles, t.nodes, t.nbr, t.hdr_key, cs, (const NodeColl *)t.coll_prev,
-                  (NodeColl *)t.coll_cur, t.counters, t.blk_cnt, mode, do_update);
+                  (const float4 *)t.tiles, t.nodes, t.nbr, t.blist, t.keys, cs, (const NodeColl *)t.coll_prev,
+                  (NodeColl *)t.coll_cur, t.counters, (const uint32_t *)t.blk_cnt, mode, do_update);
     return hipGetLastError();
 }
 
@@ -1848,9 +2113,14 @@ hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, con
                              hipStream_t stream) {
     NextTable nt;
     nt.keys = next.keys;
-    nt.vals = next.vals;
     nt.counters = next.counters;
     nt.prime = next.prime;
+    NextSort ns;
+    ns.mcount = next.mcount;
+    ns.mv_key = next.mv_key;
+    ns.mv_slot = next.mv_slot;
+    ns.stay = t.stay;
+    ns.errors = t.errors;
     Bodies b;
     b.poses = bb.poses;
     b.vels = bb.vels;
@@ -1875,15 +2145,16 @@ hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, con
                     : (P.svd_f64 ? (narrow ? k_g2p_update<15u, 128, true> : k_g2p_update<15u, 256, true>)
                                  : (narrow ? k_g2p_update<15u, 128, false> : k_g2p_update<15u, 256, false>));
     launch_kernel(ev, kern, dim3(block_grid(P, 1)), dim3(narrow ? 128 : 256), 0, stream, P, to_pbuf(in),
-                  to_pbuf(out), st, pcap, t.fdt, t.perm, t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes,
-                  t.counters, nt, b, t.hdr_key);
+                  to_pbuf(out), st, pcap, t.fdt, members_of(t), (const uint32_t *)t.blk_start,
+                  (const float4 *)t.nodes, t.keys, nt, ns, b);
     return hipGetLastError();
 }
 
 hipError_t launch_g2p(const StepParams &P, const ParticleBuffers &b, uint32_t pcap, const TransferBuffers &t,
                       const PassEvents *ev, hipStream_t stream) {
-    launch_kernel(ev, k_g2p, dim3(block_grid(P, 1)), dim3(256), 0, stream, P, to_pbuf(b), pcap, t.perm,
-                  t.blk_start, t.blk_cnt, t.nbr, (const float4 *)t.nodes, t.counters);
+    launch_kernel(ev, k_g2p, dim3(block_grid(P, 1)), dim3(256), 0, stream, P, to_pbuf(b), pcap, t.perm, t.blist,
+                  (const uint32_t *)t.blk_start, (const uint32_t *)t.blk_cnt, t.nbr, (const float4 *)t.nodes,
+                  t.counters);
     return hipGetLastError();
 }
 

