@@ -47,11 +47,11 @@ __device__ __host__ inline uint32_t tile_index(uint32_t x, uint32_t y, uint32_t 
 #define ERR_LOOKUP 2u
 #define ERR_KEY_RANGE 4u
 
-// Particle storage between substeps: every block's particles are contiguous,
-// those that stay in the block first, those that leave it last. A particle that
-// leaves (a "mover") is listed with its new block key by the substep that moved
-// it; the next substep's sort places the movers of each block in MOVER_CAP list
-// slots per block (and any beyond them in a shared overflow list).
+// Particle storage between substeps: every block's particles are contiguous. A
+// particle that leaves its block (a "mover") is listed with its new block key by
+// the substep that moved it; the next substep's sort places the movers of each
+// block in MOVER_CAP list slots per block (and any beyond them in a shared
+// overflow list).
 #define MOVER_CAP 64
 
 // Neighbour table per block (16 words): [0..6] the +1 neighbours in the
@@ -314,3 +314,4 @@ __device__ inline uint32_t run_length(uint64_t heads) {
     const uint32_t next = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : 64u;
     return next - lane;
 }
+

@@ -20,6 +20,8 @@
 //   k_g2p / k_particle_update  the same two stages as separate in-place
 //                    kernels, used when a hook replaces or observes G2P.
 //   k_rigid_update   rigid_impulses.slang:70-121.
+#include <type_traits>
+
 #include "launch.h"
 #include "../../include/slosh_hip.h"
 
@@ -1161,9 +1163,12 @@ struct BlockRange {
     bool overflow;             // some of the block's movers are in the overflow list
 };
 
+// FAST (a template parameter, so that each kernel instance loads only the
+// buffers of its mode): the steady state's stayer ranges and mover lists.
+template <bool FAST>
 __device__ inline BlockRange block_range(const Members &M, uint32_t hid) {
     BlockRange r;
-    if (M.fast) {
+    if constexpr (FAST) {
         const uint4 in = M.bin[hid];
         const uint32_t mc = M.mcount[hid];
         r.cand_lo = in.x;
@@ -1283,15 +1288,21 @@ __device__ inline uint32_t tile_global_of_lds(uint32_t j) {
     return tile_index(x, y, z);
 }
 
+// 40,848 bytes at 512 particles per chunk: four workgroups (12 waves) fit in a
+// CU's 160 KiB of LDS, as many as the registers allow.
 template <int P2G_CHUNK>
 struct P2GShared {
     static constexpr int LDS = P2G_CHUNK + P2G_CHUNK / 8;
     float4 tile[TILE_NODES];
-    float sp[16][LDS];  // m, A0[3], f[3], C[9], grouped by associated cell
+    union {
+        float sp[16][LDS];    // m, A0[3], f[3], C[9], grouped by associated cell
+        uint32_t extra[192];  // this block's overflow movers of one round: read by
+                              // p2g_chunk before it stages anything
+    };
     uint32_t lcnt[64], lstart[64];
-    uint32_t extra[192];  // this block's overflow movers of one round
     uint32_t wsum[4];
 };
+static_assert(sizeof(P2GShared<512>) <= 163840 / 4, "P2G at 512 per chunk must fit four workgroups per CU");
 
 // Phase 1 and 2 for `kn` entries of the block's sequence; entry t's storage slot
 // is slot_of(t). An entry is a member if its associated block is this block
@@ -1427,7 +1438,7 @@ __device__ inline void p2g_chunk(const StepParams &P, P2GShared<P2G_CHUNK> &S, u
     if (kz == 0) S.lcnt[lane] = 0u;
 }
 
-template <int P2G_CHUNK>
+template <int P2G_CHUNK, bool FAST>
 __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const float4 *__restrict__ pos,
                                              const float4 *__restrict__ vm, const float *__restrict__ C,
                                              const float *__restrict__ fdt, uint32_t pcap, Members M,
@@ -1446,8 +1457,8 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
         }
     }
     // Steady state: workgroup 0 computes the block totals and starts.
-    const uint32_t first = M.fast ? 1u : 0u;
-    if (M.fast && blockIdx.x == 0) {
+    const uint32_t first = FAST ? 1u : 0u;
+    if (FAST && blockIdx.x == 0) {
         p2g_block_scan(M, blk_start_out, blk_cnt_out, S.wsum);
         return;
     }
@@ -1459,7 +1470,7 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
     if (threadIdx.x < 64) S.lcnt[threadIdx.x] = 0u;
     for (uint32_t k = blockIdx.x - first; k < B; k += gridDim.x - first) {
         const uint32_t hid = M.blist[k];
-        const BlockRange br = block_range(M, hid);
+        const BlockRange br = block_range<FAST>(M, hid);
         // A block without particles (a "+1" neighbour only) has an all-zero tile:
         // nothing is written and the grid update skips it.
         if (br.total == 0) {
@@ -1737,7 +1748,7 @@ struct NextSort {
 // wavefront), so a block's particles stay together in storage. The particles
 // whose updated position leaves the block are listed, with their new block key,
 // for the next substep's sort; the others are counted (the block's stayers).
-template <uint32_t MODELS, int NT, bool F64>
+template <uint32_t MODELS, int NT, bool F64, bool FAST>
 __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_update(StepParams P, PBuf in, PBuf out, PStatic st, uint32_t pcap,
                                                     float *__restrict__ fdt, Members M,
                                                     const uint32_t *__restrict__ blk_start,
@@ -1758,7 +1769,7 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
     init.cell_cnt = nullptr;
     for (uint32_t k = blockIdx.x; k < B; k += gridDim.x) {
         const uint32_t hid = M.blist[k];
-        const BlockRange br = block_range(M, hid);
+        const BlockRange br = block_range<FAST>(M, hid);
         const uint32_t cnt = br.total;
         if (cnt == 0) {
             if (threadIdx.x == 0) ns.stay[hid] = 0u;
@@ -1782,26 +1793,8 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
         // the next table. Every lane of the workgroup calls it the same number of
         // times (the wave-level counting and run detection see all lanes).
         auto particle = [&](bool valid, uint32_t p) {
-            // Every load of the particle is issued before its membership and
-            // output slot are known (a candidate that left the block is rare).
             float4 ps = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            float F[9];
-            float dps[3] = {0.0f, 0.0f, 0.0f};
-            uint32_t oid = 0u;
-            float vm_w = 0.0f;
-            if (valid) {
-                ps = in.pos[p];
-#pragma unroll
-                for (int q = 0; q < 9; q++) F[q] = in.F[(size_t)q * pcap + p];
-                oid = in.ids[p];
-                // Uniform masses skip the 16-byte velocity / mass load (the velocity
-                // comes from the grid): only the update disables, and it zeroes the mass.
-                if (!P.mass_uniform) vm_w = in.vm[p].w;
-                if (P.has_sand) {
-#pragma unroll
-                    for (int q = 0; q < 3; q++) dps[q] = in.dps[(size_t)q * pcap + p];
-                }
-            }
+            if (valid) ps = in.pos[p];
             const bool member = valid && key_at(P.h, P.batched, ps) == block_key;
             const uint64_t lt = lanemask_lt();
             const uint64_t mm = __ballot(member);
@@ -1816,12 +1809,22 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
                 uint32_t enabled = wbits & 1u;
                 float3 pos = make_float3(ps.x, ps.y, ps.z);
                 float3 v;
-                float aff[9];
+                float aff[9], F[9];
                 float vgd;
                 g2p_particle(sh, P.h, P.inv_d, pos, enabled != 0u, &v, aff, &vgd);
-                float mass = P.mass_uniform ? (enabled ? P.u_mass : 0.0f) : vm_w;
+                // Uniform masses skip the 16-byte velocity / mass load (the velocity
+                // comes from the grid): only the update disables, and it zeroes the mass.
+                float mass = P.mass_uniform ? (enabled ? P.u_mass : 0.0f) : in.vm[p].w;
+#pragma unroll
+                for (int q = 0; q < 9; q++) F[q] = in.F[(size_t)q * pcap + p];
+                const uint32_t oid = in.ids[p];
                 const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
                 const float4 model = P.uniform ? P.u_model : st.model[oid];
+                float dps[3] = {0.0f, 0.0f, 0.0f};
+                if (P.has_sand) {
+#pragma unroll
+                    for (int q = 0; q < 3; q++) dps[q] = in.dps[(size_t)q * pcap + p];
+                }
                 update_particle<MODELS, F64>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
                 out.pos[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
                 out.vm[slot] = make_float4(v.x, v.y, v.z, mass);
@@ -2078,10 +2081,14 @@ hipError_t launch_p2g(const StepParams &P, const ParticleBuffers &pb, uint32_t p
     nt.keys = next.keys;
     nt.counters = next.counters;
     nt.prime = next.prime;
-    auto kern = P.p2g_chunk == 192u ? k_p2g<192>
-                : P.p2g_chunk == 256u ? k_p2g<256>
-                : P.p2g_chunk == 384u ? k_p2g<384>
-                                      : k_p2g<512>;
+    auto kern = t.fast ? (P.p2g_chunk == 192u   ? k_p2g<192, true>
+                          : P.p2g_chunk == 256u ? k_p2g<256, true>
+                          : P.p2g_chunk == 384u ? k_p2g<384, true>
+                                                : k_p2g<512, true>)
+                       : (P.p2g_chunk == 192u   ? k_p2g<192, false>
+                          : P.p2g_chunk == 256u ? k_p2g<256, false>
+                          : P.p2g_chunk == 384u ? k_p2g<384, false>
+                                                : k_p2g<512, false>);
     // Steady state: one more workgroup (the first) computes the block starts.
     launch_kernel(ev, kern, dim3(block_grid(P, 1) + (t.fast ? 1u : 0u)), dim3(192), 0, stream, P, nt,
                   (const float4 *)pb.pos, (const float4 *)pb.vm, (const float *)pb.C, (const float *)t.fdt, pcap,
@@ -2138,12 +2145,16 @@ hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, con
     const bool nh_only = (P.tag_mask & ~2u) == 0u;
     const bool elastic_only = (P.tag_mask & ~3u) == 0u;
     const bool narrow = P.g2p_threads == 128u;
-    auto kern = nh_only ? (narrow ? k_g2p_update<2u, 128, false> : k_g2p_update<2u, 256, false>)
-                : elastic_only
-                    ? (P.svd_f64 ? (narrow ? k_g2p_update<3u, 128, true> : k_g2p_update<3u, 256, true>)
-                                 : (narrow ? k_g2p_update<3u, 128, false> : k_g2p_update<3u, 256, false>))
-                    : (P.svd_f64 ? (narrow ? k_g2p_update<15u, 128, true> : k_g2p_update<15u, 256, true>)
-                                 : (narrow ? k_g2p_update<15u, 128, false> : k_g2p_update<15u, 256, false>));
+    auto pick = [&](auto fast) {
+        constexpr bool F = decltype(fast)::value;
+        return nh_only ? (narrow ? k_g2p_update<2u, 128, false, F> : k_g2p_update<2u, 256, false, F>)
+               : elastic_only
+                   ? (P.svd_f64 ? (narrow ? k_g2p_update<3u, 128, true, F> : k_g2p_update<3u, 256, true, F>)
+                                : (narrow ? k_g2p_update<3u, 128, false, F> : k_g2p_update<3u, 256, false, F>))
+                   : (P.svd_f64 ? (narrow ? k_g2p_update<15u, 128, true, F> : k_g2p_update<15u, 256, true, F>)
+                                : (narrow ? k_g2p_update<15u, 128, false, F> : k_g2p_update<15u, 256, false, F>));
+    };
+    auto kern = t.fast ? pick(std::true_type{}) : pick(std::false_type{});
     launch_kernel(ev, kern, dim3(block_grid(P, 1)), dim3(narrow ? 128 : 256), 0, stream, P, to_pbuf(in),
                   to_pbuf(out), st, pcap, t.fdt, members_of(t), (const uint32_t *)t.blk_start,
                   (const float4 *)t.nodes, t.keys, nt, ns, b);
