@@ -1787,24 +1787,36 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
             n_stay = 0u;
         }
         __syncthreads();
-        // One entry of the block's sequence (a candidate is a member only if it is
-        // still in the block): G2P, update, output at the next free slot of the
-        // block's range, listed if it leaves the block, its blocks inserted into
-        // the next table. Every lane of the workgroup calls it the same number of
-        // times (the wave-level counting and run detection see all lanes).
-        auto particle = [&](bool valid, uint32_t p) {
+        // The block's entries (a candidate is a member only if it is still in the
+        // block), then, only if some of its movers overflowed their list, the
+        // overflow list, whose entries of other blocks are skipped. Per member:
+        // G2P, update, output at the next free slot of the block's range (one LDS
+        // atomic per wavefront), listed if it leaves the block, its blocks
+        // inserted into the next table. Every lane of the workgroup runs the same
+        // number of iterations (the wave-level counting and run detection see all
+        // lanes).
+        const uint32_t n_main = br.cand_n + br.list_n;
+        const uint32_t n_all = n_main + (br.overflow ? n_ovf : 0u);
+        for (uint32_t jb = 0; jb < n_all; jb += blockDim.x) {
+            const uint32_t m = jb + threadIdx.x;
+            bool entry = m < n_all;
+            uint32_t p = 0u;
+            if (m < n_main) {
+                p = entry_slot(br, m);
+            } else if (entry) {
+                entry = M.ovf_hid[m - n_main] == hid;
+                p = M.ovf_slot[m - n_main];
+            }
             float4 ps = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (valid) ps = in.pos[p];
-            const bool member = valid && key_at(P.h, P.batched, ps) == block_key;
-            const uint64_t lt = lanemask_lt();
-            const uint64_t mm = __ballot(member);
+            if (entry) ps = in.pos[p];
+            const bool valid = entry && key_at(P.h, P.batched, ps) == block_key;
+            const uint64_t vmask = __ballot(valid);
             uint32_t ob = 0u;
-            if (lane == 0 && mm) ob = atomicAdd(&n_out, (uint32_t)__popcll(mm));
-            const uint32_t j = __shfl(ob, 0) + (uint32_t)__popcll(mm & lt);
+            if (lane == 0 && vmask) ob = atomicAdd(&n_out, (uint32_t)__popcll(vmask));
+            const uint32_t j = __shfl(ob, 0) + (uint32_t)__popcll(vmask & lanemask_lt());
             const uint32_t slot = s0 + j;
             uint32_t next_key = 0u;
-            bool in_range = true;
-            if (member && j < cnt) {
+            if (valid) {
                 const uint32_t wbits = __float_as_uint(ps.w);
                 uint32_t enabled = wbits & 1u;
                 float3 pos = make_float3(ps.x, ps.y, ps.z);
@@ -1844,57 +1856,46 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
                     fdt[(size_t)pcap + slot] = 0.0f;
                     fdt[2 * (size_t)pcap + slot] = 0.0f;
                 }
-                int nx, ny, nz;
-                uint32_t cell;
-                assoc_block_cell(P.h, pos.x, pos.y, pos.z, &nx, &ny, &nz, &cell);
-                next_key = pack_key(P.batched, wbits >> 1, nx, ny, nz);
-                in_range = block_in_key_range(P.batched, nx, ny, nz);
+                next_key = key_at(P.h, P.batched, make_float4(pos.x, pos.y, pos.z, __uint_as_float(wbits)));
             }
-            // A block outside the key space would alias another one: reported,
-            // once per wavefront.
-            const uint64_t bad = __ballot(!in_range);
-            if (bad && lane == (uint32_t)(__ffsll((unsigned long long)bad) - 1)) atomicOr(ns.errors, ERR_KEY_RANGE);
-            const bool mover = member && j < cnt && next_key != block_key;
-            const bool stay = member && j < cnt && !mover;
-            stays |= stay;
-            const uint64_t sm = __ballot(stay);
+            const bool mover = valid && next_key != block_key;
+            stays |= valid && !mover;
+            const uint64_t sm = __ballot(valid && !mover);
             if (lane == 0 && sm) atomicAdd(&n_stay, (uint32_t)__popcll(sm));
-            if (nt.prime) {
-                // The movers, listed for the next substep's sort.
-                const uint64_t vm_ = __ballot(mover);
-                if (vm_) {
+            const uint64_t mv = __ballot(mover);
+            if (mv) {
+                // Only a particle that leaves its block can leave the key space (a
+                // block outside it would alias another one): reported, once per
+                // wavefront.
+                bool out_of_range = false;
+                if (mover) {
+                    const float4 np = out.pos[slot];
+                    int nx, ny, nz;
+                    uint32_t cell;
+                    assoc_block_cell(P.h, np.x, np.y, np.z, &nx, &ny, &nz, &cell);
+                    out_of_range = !block_in_key_range(P.batched, nx, ny, nz);
+                }
+                if (lane == 0 && __ballot(out_of_range)) atomicOr(ns.errors, ERR_KEY_RANGE);
+                if (nt.prime) {
+                    // The movers, listed for the next substep's sort.
                     uint32_t lb = 0u;
-                    if (lane == 0) lb = atomicAdd(&nt.counters[CTR_MOVERS], (uint32_t)__popcll(vm_));
+                    if (lane == 0) lb = atomicAdd(&nt.counters[CTR_MOVERS], (uint32_t)__popcll(mv));
                     lb = __shfl(lb, 0);
                     if (mover) {
-                        const uint32_t e = lb + (uint32_t)__popcll(vm_ & lt);
+                        const uint32_t e = lb + (uint32_t)__popcll(mv & lanemask_lt());
                         ns.mv_key[e] = next_key;
                         ns.mv_slot[e] = slot;
                     }
                 }
+            }
+            if (nt.prime) {
                 // touch_primary_blocks and touch_neighbor_blocks of the next
                 // substep (sort.slang:50-92). Particles that stay in this block
                 // only flag it (its 8 blocks are inserted once below); the others
                 // insert theirs, one round per run of equal keys.
-                const uint64_t heads = run_heads(next_key, mover) & vm_;
+                const uint64_t heads = run_heads(next_key, mover) & mv;
                 insert_blocks(nt.keys, P.cap, P.batched, next_key, heads, nt.counters, init);
             }
-        };
-        // The block's entries, then (only if some of its movers overflowed their
-        // list) the overflow list, whose entries of other blocks are skipped.
-        const uint32_t n_main = br.cand_n + br.list_n;
-        const uint32_t n_all = n_main + (br.overflow ? n_ovf : 0u);
-        for (uint32_t jb = 0; jb < n_all; jb += blockDim.x) {
-            const uint32_t m = jb + threadIdx.x;
-            bool valid = m < n_all;
-            uint32_t p = 0u;
-            if (m < n_main) {
-                p = entry_slot(br, m);
-            } else if (valid) {
-                valid = M.ovf_hid[m - n_main] == hid;
-                p = M.ovf_slot[m - n_main];
-            }
-            particle(valid, p);
         }
         const bool any_stays = __syncthreads_or(stays);
         if (threadIdx.x == 0) ns.stay[hid] = n_stay;

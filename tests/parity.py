@@ -67,3 +67,10 @@ def tol_for(steps):
     """Relative L2 bound after `steps` substeps: 1e-5 after one substep and 1e-3
     after 100 (the parity gates), linear in between."""
     return 1.0e-5 * min(max(int(steps), 1), 100)
+
+
+def tol_long(steps):
+    """The same linear growth beyond 100 substeps (1e-5 per substep), for runs of
+    hundreds to thousands of substeps (served requests): two evaluations that
+    differ by round-off keep drifting apart, the APIC matrix fastest."""
+    return 1.0e-5 * max(int(steps), 1)

@@ -12,7 +12,7 @@ import pytest
 from slosh_amd import pipeline as P
 from slosh_amd import scenes
 from slosh_amd import serving as SV
-from tests.parity import state_errors, tol_for
+from tests.parity import state_errors, tol_for, tol_long
 
 pytestmark = pytest.mark.gpu
 
@@ -84,7 +84,9 @@ def test_served_stream_at_full_request_sizes(gpu_pipeline):
     (50 req/s on a clock of 1000 substeps per second) under the 200 GB budget.
     Every request: finite, none disabled, masses kept bit for bit, and the same
     result alone (its own MpmData, captured graphs) as in the mix. The requests
-    with at most 2e7 particle-substeps: also against the oracle."""
+    with at most 2e7 particle-substeps: also against the oracle. Requests run
+    182-1429 substeps: the gate grows linearly with the substep count beyond
+    100, at the rate of the parity gates (1e-5 per substep)."""
     from oracle import oracle as O
     pl, st = gpu_pipeline
     arr = scenes.poisson_arrivals(50.0, 16) * 1000.0
@@ -118,7 +120,7 @@ def test_served_stream_at_full_request_sizes(gpu_pipeline):
               " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
         assert errs["enabled_diff"] == 0
         for k in ("x", "v", "affine", "F"):
-            assert errs[k] <= tol_for(r.substeps), f"request {r.rid} {k}: {errs[k]:.3e}"
+            assert errs[k] <= tol_long(r.substeps), f"request {r.rid} {k}: {errs[k]:.3e}"
         if r.n * r.substeps <= 2.0e7:
             orc = O.Oracle(r.scene, threads=16)
             orc.step(r.substeps)
@@ -127,6 +129,6 @@ def test_served_stream_at_full_request_sizes(gpu_pipeline):
             print(f"  vs oracle: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
             assert errs["enabled_diff"] == 0
             for k in ("x", "v", "affine", "F"):
-                assert errs[k] <= tol_for(r.substeps), f"request {r.rid} vs oracle {k}: {errs[k]:.3e}"
+                assert errs[k] <= tol_long(r.substeps), f"request {r.rid} vs oracle {k}: {errs[k]:.3e}"
             checked += 1
     assert checked >= 4
