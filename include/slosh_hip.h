@@ -263,6 +263,16 @@ int slosh_set_sim_params(slosh_data *data, const slosh_sim_params *params);
  * (computed_dt of the u32 fixed point in 1e-12 s units, also returned in raw if
  * non-null). Synchronous on `stream`. */
 int slosh_timestep_bound(slosh_data *data, slosh_stream stream, float *out_dt, uint32_t *out_raw);
+/* No counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops). slosh_timestep_bound's u32 fixed-point
+ * bound, bit-identical, written to DEVICE memory `out_raw` (0xffffffff with no
+ * enabled particle), asynchronous on `stream`: no allocation, no memset, no copy,
+ * no synchronization. Its partial minima (one per 2048 particles) live in a
+ * static 64 KiB array per GPU, so at most SLOSH_NOALLOC_MAX_PARTICLES particles
+ * (else SLOSH_E_INVALID). Every call on one GPU shares that array: calls that
+ * overlap on one GPU (different streams, no ordering between them) race. */
+#define SLOSH_NOALLOC_MAX_PARTICLES 33554432u
+int slosh_timestep_bound_noalloc(slosh_data *data, slosh_stream stream, uint32_t *out_raw);
 /* The adaptive substep count of src_testbed/step.rs:100-147: with
  * min_substeps < max_substeps, ceil(base_dt / bound) clamped to the range;
  * otherwise max_substeps. Sets the simulation dt to base_dt / count. */

@@ -54,6 +54,9 @@ __device__ __host__ inline uint32_t tile_index(uint32_t x, uint32_t y, uint32_t 
 // overflow list).
 #define MOVER_CAP 64
 
+// Particles per workgroup of the two-pass timestep bound (one partial minimum each).
+#define TSB_CHUNK 2048u
+
 // Neighbour table per block (16 words): [0..6] the +1 neighbours in the
 // reference's associated-block order (grid.slang:371-379, offsets 1..7),
 // [7] the block's header id in the previous substep's hash map,

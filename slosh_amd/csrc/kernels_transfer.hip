@@ -1921,38 +1921,77 @@ __device__ inline uint32_t secs_to_int(float secs) {
     return (uint32_t)x;
 }
 
+// The bound of particle i (0xffffffff for a disabled particle).
+__device__ inline uint32_t particle_dt_bound(const StepParams &P, const PBuf &buf, const PStatic &st, uint32_t pcap,
+                                             uint32_t i) {
+#pragma clang fp contract(off)
+    const float h = P.h;
+    const float4 ps = buf.pos[i];
+    if ((__float_as_uint(ps.w) & 1u) == 0u) return 0xffffffffu;
+    const float4 vm = buf.vm[i];
+    const uint32_t oid = buf.ids[i];
+    const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
+    const float4 model = P.uniform ? P.u_model : st.model[oid];
+    float F[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) F[k] = buf.F[(size_t)k * pcap + i];
+    const float speed = sqrtf(vm.x * vm.x + vm.y * vm.y + vm.z * vm.z);
+    float dt = 0.0f;
+    if (__float_as_uint(model.x) < 4u) {
+        const float lambda = model.y, mu = model.z, cfl = model.w;
+        const float bulk = lambda + 2.0f * mu / 3.0f;
+        const float shear = mu;
+        const float density0 = vm.w / prop.x;
+        const float curr_density = density0 / fmaxf(det3(F), 1.0e-6f);
+        const float sound_speed = sqrtf((bulk + shear * 4.0f / 3.0f) / curr_density);
+        dt = cfl * h / fmaxf(speed, sound_speed);
+    }
+    dt = fminf(dt, h / speed);
+    return secs_to_int(dt);
+}
+
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+
 __global__ __launch_bounds__(256) void k_timestep_bound(StepParams P, PBuf buf, PStatic st, uint32_t pcap,
                                                         uint32_t *__restrict__ out) {
-#pragma clang fp contract(off)
     uint32_t best = 0xffffffffu;
-    const float h = P.h;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += gridDim.x * blockDim.x) {
-        const float4 ps = buf.pos[i];
-        if ((__float_as_uint(ps.w) & 1u) == 0u) continue;
-        const float4 vm = buf.vm[i];
-        const uint32_t oid = buf.ids[i];
-        const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
-        const float4 model = P.uniform ? P.u_model : st.model[oid];
-        float F[9];
-#pragma unroll
-        for (int k = 0; k < 9; k++) F[k] = buf.F[(size_t)k * pcap + i];
-        const float speed = sqrtf(vm.x * vm.x + vm.y * vm.y + vm.z * vm.z);
-        float dt = 0.0f;
-        if (__float_as_uint(model.x) < 4u) {
-            const float lambda = model.y, mu = model.z, cfl = model.w;
-            const float bulk = lambda + 2.0f * mu / 3.0f;
-            const float shear = mu;
-            const float density0 = vm.w / prop.x;
-            const float curr_density = density0 / fmaxf(det3(F), 1.0e-6f);
-            const float sound_speed = sqrtf((bulk + shear * 4.0f / 3.0f) / curr_density);
-            dt = cfl * h / fmaxf(speed, sound_speed);
-        }
-        dt = fminf(dt, h / speed);
-        best = min(best, secs_to_int(dt));
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) best = min(best, (uint32_t)__shfl_xor((int)best, o));
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += gridDim.x * blockDim.x)
+        best = min(best, particle_dt_bound(P, buf, st, pcap, i));
+    best = wave_min_u32(best);
     if (lane_id() == 0 && best != 0xffffffffu) atomicMin(out, best);
+}
+
+// The same bound in two passes with no atomics and nothing to clear first:
+// workgroup g stores the minimum over particles [g * TSB_CHUNK, (g + 1) *
+// TSB_CHUNK) into partials[g] (0xffffffff if none is enabled), then one
+// workgroup stores the minimum of the partials into *out. Integer minima, so
+// the result is bit-identical to k_timestep_bound's.
+__global__ __launch_bounds__(256) void k_timestep_bound_partials(StepParams P, PBuf buf, PStatic st, uint32_t pcap,
+                                                                 uint32_t *__restrict__ partials) {
+    __shared__ uint32_t wmin[4];
+    uint32_t best = 0xffffffffu;
+    const uint32_t base = blockIdx.x * TSB_CHUNK;
+    const uint32_t end = min(P.n, base + TSB_CHUNK);
+    for (uint32_t i = base + threadIdx.x; i < end; i += 256) best = min(best, particle_dt_bound(P, buf, st, pcap, i));
+    best = wave_min_u32(best);
+    if (lane_id() == 0) wmin[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
+}
+
+__global__ __launch_bounds__(256) void k_min_partials(const uint32_t *__restrict__ partials, uint32_t count,
+                                                      uint32_t *__restrict__ out) {
+    __shared__ uint32_t wmin[4];
+    uint32_t best = 0xffffffffu;
+    for (uint32_t i = threadIdx.x; i < count; i += 256) best = min(best, partials[i]);
+    best = wave_min_u32(best);
+    if (lane_id() == 0) wmin[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
 }
 
 // G2P only, in place (hook path): writes velocity, velocity gradient, vel_grad_det.
@@ -2194,6 +2233,21 @@ hipError_t launch_timestep_bound(const StepParams &P, const ParticleBuffers &b, 
     st.dp1 = s.dp1;
     hipLaunchKernelGGL(k_timestep_bound, dim3(min_t(div_up_t(P.n, 256), 4096)), dim3(256), 0, stream, P, to_pbuf(b),
                        st, pcap, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_timestep_bound_partials(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s,
+                                          uint32_t pcap, uint32_t *partials, uint32_t *out, hipStream_t stream) {
+    PStatic st;
+    st.prop = s.prop;
+    st.model = s.model;
+    st.dp0 = s.dp0;
+    st.dp1 = s.dp1;
+    const uint32_t groups = div_up_t(P.n, TSB_CHUNK);
+    if (groups > 0)
+        hipLaunchKernelGGL(k_timestep_bound_partials, dim3(groups), dim3(256), 0, stream, P, to_pbuf(b), st, pcap,
+                           partials);
+    hipLaunchKernelGGL(k_min_partials, dim3(1), dim3(256), 0, stream, partials, groups, out);
     return hipGetLastError();
 }
 

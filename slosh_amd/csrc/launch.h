@@ -90,6 +90,11 @@ hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b,
 hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, const PassEvents *ev, hipStream_t stream);
 hipError_t launch_timestep_bound(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
                                  uint32_t *out, hipStream_t stream);
+// Two passes over partial minima, one per TSB_CHUNK particles, in `partials`
+// (div_up(n, TSB_CHUNK) words); *out = the bound, 0xffffffff with no enabled
+// particle. Neither buffer needs clearing first.
+hipError_t launch_timestep_bound_partials(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s,
+                                          uint32_t pcap, uint32_t *partials, uint32_t *out, hipStream_t stream);
 
 // ---- particle storage (kernels_particles.hip) ------------------------------------
 struct CompactArgs {

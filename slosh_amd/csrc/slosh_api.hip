@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -118,6 +119,7 @@ struct slosh_graph {
 
 struct slosh_data {
     slosh_pipeline *pipeline = nullptr;
+    int device = -1;  // the device current at creation, which holds every buffer
     size_t n = 0, pcap = 0;
     uint32_t cap = 0;
     float h = 0.0f;
@@ -1157,6 +1159,38 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     return SLOSH_OK;
 }
 
+// ---- Scratch of slosh_timestep_bound_noalloc ------------------------------
+// One partial minimum per TSB_CHUNK particles: NOALLOC_PARTIALS words (64 KiB)
+// on every GPU the library runs on, held until the process ends, shared by all
+// calls on that GPU. In host code `scratch` is a host-side stand-in, not a
+// device address: kernels get the address hipGetSymbolAddress returns for the
+// device (noalloc_scratch[device]), resolved once per device at slosh_data_create
+// with that device current.
+#define NOALLOC_PARTIALS 16384u
+static_assert((size_t)NOALLOC_PARTIALS * TSB_CHUNK == SLOSH_NOALLOC_MAX_PARTICLES, "scratch size");
+#define NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) uint32_t scratch[NOALLOC_PARTIALS];
+static std::mutex noalloc_mu;
+static uint32_t *noalloc_scratch[NOALLOC_MAX_DEVICES] = {};
+
+static int resolve_noalloc_scratch(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(SLOSH_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (noalloc_scratch[device]) return SLOSH_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(SLOSH_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    noalloc_scratch[device] = (uint32_t *)ptr;
+    return SLOSH_OK;
+}
+
+static uint32_t *noalloc_scratch_of(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    return noalloc_scratch[device];
+}
+
 int slosh_data_create(slosh_pipeline *pipeline, const slosh_sim_params *params, const slosh_particles *particles,
                       size_t particle_capacity, const slosh_body_desc *bodies, size_t num_bodies,
                       const slosh_boundary_condition *body_materials, float cell_width, uint32_t grid_capacity,
@@ -1164,8 +1198,11 @@ int slosh_data_create(slosh_pipeline *pipeline, const slosh_sim_params *params, 
     if (!pipeline || !out) return fail(SLOSH_E_INVALID, "null pipeline or output");
     *out = nullptr;
     slosh_data *d = new slosh_data();
-    int rc = create_impl(pipeline, params, particles, particle_capacity, bodies, num_bodies, body_materials, cell_width,
-                         grid_capacity, d);
+    int rc = create_impl(pipeline, params, particles, particle_capacity, bodies, num_bodies, body_materials,
+                         cell_width, grid_capacity, d);
+    if (rc == SLOSH_OK)
+        rc = hipGetDevice(&d->device) == hipSuccess ? resolve_noalloc_scratch(d->device)
+                                                    : fail(SLOSH_E_HIP, "hipGetDevice failed");
     if (rc != SLOSH_OK) {
         std::string msg = g_last_error;
         free_data(d);
@@ -1396,6 +1433,21 @@ int slosh_timestep_bound(slosh_data *d, slosh_stream stream, float *out_dt, uint
     HIP_TRY(hipStreamSynchronize(st));
     *out_dt = (float)raw / 1.0e12f;  // GpuTimestepBounds::computed_dt
     if (out_raw) *out_raw = raw;
+    return SLOSH_OK;
+}
+
+int slosh_timestep_bound_noalloc(slosh_data *d, slosh_stream stream, uint32_t *out_raw) {
+    if (!d || !out_raw) return fail(SLOSH_E_INVALID, "null argument");
+    if (d->n > (size_t)NOALLOC_PARTIALS * TSB_CHUNK)
+        return fail(SLOSH_E_INVALID, "slosh_timestep_bound_noalloc: " + std::to_string(d->n) +
+                                         " particles exceed the " +
+                                         std::to_string((size_t)NOALLOC_PARTIALS * TSB_CHUNK) +
+                                         " its static scratch holds (use slosh_timestep_bound)");
+    uint32_t *ptr = noalloc_scratch_of(d->device);
+    if (!ptr) return fail(SLOSH_E_HIP, "slosh_timestep_bound_noalloc: scratch address not resolved for the device");
+    const StepParams P = make_params(d);
+    HIP_TRY(launch_timestep_bound_partials(P, pbufs(d, d->cur_p), sbufs(d), (uint32_t)d->pcap, ptr, out_raw,
+                                           (hipStream_t)stream));
     return SLOSH_OK;
 }
 

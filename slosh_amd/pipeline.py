@@ -291,6 +291,11 @@ class MpmData:
         _lib.check(_lib.load().slosh_timestep_bound(self.handle, self._s(stream), C.byref(dt), C.byref(raw)))
         return dt.value, raw.value
 
+    def timestep_bound_noalloc(self, out_raw_ptr, stream=None):
+        """The raw bound into device memory at `out_raw_ptr` (one u32), asynchronous,
+        allocating nothing (static per-GPU scratch; at most 33554432 particles)."""
+        _lib.check(_lib.load().slosh_timestep_bound_noalloc(self.handle, self._s(stream), C.c_void_p(out_raw_ptr)))
+
     def adaptive_substeps(self, base_dt, min_substeps, max_substeps, stream=None):
         """Substep count for one frame of length base_dt (src_testbed/step.rs:100-147);
         sets the simulation dt to base_dt / count."""
