@@ -61,15 +61,19 @@ def pass_bytes(n, n_blk, n_dp=0):
 
 
 def measured_traffic(workload, pass_name, tuning):
-    """HBM bytes per launch of the kernel instance `pass_name` runs under `tuning`,
-    from the committed PMC summary (the dam break: neo-Hookean only, plain SVD path)."""
-    if workload != "dam_break":
+    """HBM bytes per launch of the kernel instance `pass_name` runs under `tuning`
+    (steady-state substeps), from the committed PMC summary of the workload: the
+    dam break runs the neo-Hookean-only instance, the batch the all-materials
+    instance with the refined SVD path."""
+    files = {"dam_break": "dam_break_1M_hbm_traffic.md", "batched": "batched_256_hbm_traffic.md"}
+    if workload not in files:
         return None
-    path = os.path.join(ROOT, "profiles", "dam_break_1M_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", files[workload])
     if not os.path.exists(path):
         return None
-    kernel = {"g2p": "k_g2p_update<2u,%d,false>" % tuning["g2p_threads"],
-              "p2g": "k_p2g<%d>" % tuning["p2g_chunk"], "grid_update": "k_grid_update<false>"}.get(pass_name)
+    g2p = "k_g2p_update<2u,%d,false,true>" if workload == "dam_break" else "k_g2p_update<15u,%d,true,true>"
+    kernel = {"g2p": g2p % tuning["g2p_threads"], "p2g": "k_p2g<%d,true>" % tuning["p2g_chunk"],
+              "grid_update": "k_grid_update<false>"}.get(pass_name)
     for ln in open(path):
         f = ln.split()
         if len(f) == 4 and f[0] == "traffic" and f[1] == pass_name and f[2] == kernel:
@@ -216,7 +220,25 @@ def run_batched(args):
     n_blk = data.num_blocks(st)
     batched = batch.n * args.batch_steps / (ms * 1e-3)
     g = None
+    # Roofline of the batch's dominant kernel, as for the single scene: more
+    # substeps launched directly with packet-written events around every pass.
+    ts = P.Timestamps()
+    ts.reset()
+    rsteps = max(2, min(args.steps, 20))
+    for _ in range(rsteps):
+        pl.launch_step(data, st, timestamps=ts)
+    per_pass = ts.read()
+    data.check_errors(st)
     data.close()
+    tags = batch.particles["models"][:, 0]
+    n_dp = int(np.count_nonzero((tags == 2) | (tags == 3)))
+    nbytes = pass_bytes(batch.n, n_blk, n_dp)
+    passes = {k: {"mean_us": 1e3 * v[0] / max(v[1], 1), "launches": v[1]} for k, v in per_pass.items() if v[1]}
+    for k, v in passes.items():
+        if k in nbytes:
+            v["alg_bytes"] = nbytes[k]
+            v["GBps"] = nbytes[k] / (v["mean_us"] * 1e-6) / 1e9
+    dom = max((k for k in passes if k in nbytes and k != "grid_sort"), key=lambda k: passes[k]["mean_us"])
     out = {
         "workload": f"{len(scs)} scenes, {batch.n} particles (N log-uniform [2e3, 2e5]; NH / FC / DP)",
         "value": batched,
@@ -224,7 +246,18 @@ def run_batched(args):
         "steps": args.batch_steps,
         "ms_per_step": ms / args.batch_steps,
         "active_blocks": int(n_blk),
+        "drucker_prager_particles": n_dp,
         "tuning": tuning,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": passes[dom]["GBps"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": passes[dom]["GBps"] / HBM_PEAK_GBS,
+            "traffic": measured_traffic("batched", dom, tuning),
+        },
+        "passes": passes,
     }
     if not args.batch_loop:
         return out

@@ -1808,7 +1808,20 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
                 p = M.ovf_slot[m - n_main];
             }
             float4 ps = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (entry) ps = in.pos[p];
+            // The neo-Hookean-only instance has registers to spare: it issues the
+            // deformation gradient and id loads with the position load (nearly
+            // every entry is a member), the others after the G2P sums.
+            constexpr bool EARLY = MODELS == 2u;
+            float F[9];
+            uint32_t oid = 0u;
+            if (entry) {
+                ps = in.pos[p];
+                if (EARLY) {
+#pragma unroll
+                    for (int q = 0; q < 9; q++) F[q] = in.F[(size_t)q * pcap + p];
+                    oid = in.ids[p];
+                }
+            }
             const bool valid = entry && key_at(P.h, P.batched, ps) == block_key;
             const uint64_t vmask = __ballot(valid);
             uint32_t ob = 0u;
@@ -1821,15 +1834,17 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
                 uint32_t enabled = wbits & 1u;
                 float3 pos = make_float3(ps.x, ps.y, ps.z);
                 float3 v;
-                float aff[9], F[9];
+                float aff[9];
                 float vgd;
                 g2p_particle(sh, P.h, P.inv_d, pos, enabled != 0u, &v, aff, &vgd);
                 // Uniform masses skip the 16-byte velocity / mass load (the velocity
                 // comes from the grid): only the update disables, and it zeroes the mass.
                 float mass = P.mass_uniform ? (enabled ? P.u_mass : 0.0f) : in.vm[p].w;
+                if (!EARLY) {
 #pragma unroll
-                for (int q = 0; q < 9; q++) F[q] = in.F[(size_t)q * pcap + p];
-                const uint32_t oid = in.ids[p];
+                    for (int q = 0; q < 9; q++) F[q] = in.F[(size_t)q * pcap + p];
+                    oid = in.ids[p];
+                }
                 const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
                 const float4 model = P.uniform ? P.u_model : st.model[oid];
                 float dps[3] = {0.0f, 0.0f, 0.0f};
