@@ -84,6 +84,9 @@ struct StepParams {
     uint32_t tag_mask;    // material tags present (bit t = tag t; bit 4 = any other tag)
     uint32_t uniform;     // every particle has the same static record: use u_* below
     float4 u_prop, u_model, u_dp0, u_dp1;
+    // Batched data whose every scene has one static record: read it from the
+    // per-scene table (scene id from the position's w bits) instead of by id.
+    uint32_t scene_uniform;
     // Every particle was created enabled with mass u_mass, and only the update has
     // disabled particles since (it zeroes their mass): mass = enabled ? u_mass : 0.
     uint32_t mass_uniform;

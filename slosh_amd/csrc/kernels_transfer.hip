@@ -992,7 +992,38 @@ struct PStatic {
     const float4 *model; // by original id: (tag bits, lambda, mu, cfl) of the elastic part
     const float4 *dp0;   // by original id: (h0, h1, h2, h3)
     const float4 *dp1;   // by original id: (lambda, mu, 0, 0) of the plastic part
+    const float4 *scene_rec;  // batched, one record per scene: prop, model, dp0, dp1 per scene id
 };
+
+// A particle's static records: the single record, its scene's record, or its own
+// (by original id).
+__device__ inline void static_records(const StepParams &P, const PStatic &st, uint32_t oid, uint32_t scene,
+                                      float4 *prop, float4 *model) {
+    if (P.uniform) {
+        *prop = P.u_prop;
+        *model = P.u_model;
+    } else if (P.scene_uniform) {
+        *prop = st.scene_rec[4 * (size_t)scene];
+        *model = st.scene_rec[4 * (size_t)scene + 1];
+    } else {
+        *prop = st.prop[oid];
+        *model = st.model[oid];
+    }
+}
+
+__device__ inline void plastic_records(const StepParams &P, const PStatic &st, uint32_t oid, uint32_t scene,
+                                       float4 *dp0, float4 *dp1) {
+    if (P.uniform) {
+        *dp0 = P.u_dp0;
+        *dp1 = P.u_dp1;
+    } else if (P.scene_uniform) {
+        *dp0 = st.scene_rec[4 * (size_t)scene + 2];
+        *dp1 = st.scene_rec[4 * (size_t)scene + 3];
+    } else {
+        *dp0 = st.dp0[oid];
+        *dp1 = st.dp1[oid];
+    }
+}
 
 // G2P of one particle from the 6x6x6 LDS halo (g2p.slang:150-265):
 // v = sum w v_i, C = (4/h^2) sum w v_i dpt_i^T, vel_grad_det = (4/h^2) sum w v_i . dpt_i.
@@ -1073,7 +1104,7 @@ __device__ inline void g2p_particle(const float4 *__restrict__ sh, float h, floa
 template <uint32_t MODELS, bool F64>
 __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 &v, float &mass, uint32_t &enabled,
                                        float *aff, float *F, float4 prop, float4 model, const PStatic &st,
-                                       uint32_t oid, float *dps) {
+                                       uint32_t oid, uint32_t scene, float *dps) {
 #pragma clang fp contract(off)
     const float dt = P.dt, h = P.h;
     float vl = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
@@ -1106,8 +1137,8 @@ __device__ inline void update_particle(const StepParams &P, float3 &pos, float3 
         if (F64) fc_stress_f64(model.y, model.z, F, stress);
         else fc_stress(model.y, model.z, F, stress);
     } else if (((MODELS & 4u) && tag == 2u) || ((MODELS & 8u) && tag == 3u)) {
-        float4 hp = P.uniform ? P.u_dp0 : st.dp0[oid];
-        float4 pl = P.uniform ? P.u_dp1 : st.dp1[oid];
+        float4 hp, pl;
+        plastic_records(P, st, oid, scene, &hp, &pl);
         if (F64) dp_project_f64(dps, hp, pl.x, pl.y, F);
         else dp_project(dps, hp, pl.x, pl.y, F);
         if (tag == 3u) nh_stress(model.y, model.z, F, stress);
@@ -1845,14 +1876,15 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
                     for (int q = 0; q < 9; q++) F[q] = in.F[(size_t)q * pcap + p];
                     oid = in.ids[p];
                 }
-                const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
-                const float4 model = P.uniform ? P.u_model : st.model[oid];
+                const uint32_t scene = wbits >> 1;
+                float4 prop, model;
+                static_records(P, st, oid, scene, &prop, &model);
                 float dps[3] = {0.0f, 0.0f, 0.0f};
                 if (P.has_sand) {
 #pragma unroll
                     for (int q = 0; q < 3; q++) dps[q] = in.dps[(size_t)q * pcap + p];
                 }
-                update_particle<MODELS, F64>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
+                update_particle<MODELS, F64>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, scene, dps);
                 out.pos[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
                 out.vm[slot] = make_float4(v.x, v.y, v.z, mass);
 #pragma unroll
@@ -1945,8 +1977,8 @@ __device__ inline uint32_t particle_dt_bound(const StepParams &P, const PBuf &bu
     if ((__float_as_uint(ps.w) & 1u) == 0u) return 0xffffffffu;
     const float4 vm = buf.vm[i];
     const uint32_t oid = buf.ids[i];
-    const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
-    const float4 model = P.uniform ? P.u_model : st.model[oid];
+    float4 prop, model;
+    static_records(P, st, oid, __float_as_uint(ps.w) >> 1, &prop, &model);
     float F[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) F[k] = buf.F[(size_t)k * pcap + i];
@@ -2066,9 +2098,9 @@ __global__ __launch_bounds__(256, F64 ? 3 : 1) void k_particle_update(StepParams
 #pragma unroll
         for (int k = 0; k < 3; k++) dps[k] = buf.dps[(size_t)k * pcap + p];
     }
-    const float4 prop = P.uniform ? P.u_prop : st.prop[oid];
-    const float4 model = P.uniform ? P.u_model : st.model[oid];
-    update_particle<MODELS, F64>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, dps);
+    float4 prop, model;
+    static_records(P, st, oid, wbits >> 1, &prop, &model);
+    update_particle<MODELS, F64>(P, pos, v, mass, enabled, aff, F, prop, model, st, oid, wbits >> 1, dps);
     buf.pos[p] = make_float4(pos.x, pos.y, pos.z, __uint_as_float((wbits & ~1u) | enabled));
     buf.vm[p] = make_float4(v.x, v.y, v.z, mass);
 #pragma unroll
@@ -2195,6 +2227,7 @@ hipError_t launch_g2p_update(const StepParams &P, const ParticleBuffers &in, con
     st.model = s.model;
     st.dp0 = s.dp0;
     st.dp1 = s.dp1;
+    st.scene_rec = s.scene_rec;
     // The lightest instance that handles every material present: neo-Hookean
     // only (no SVD code), fixed-corotated and neo-Hookean, every material.
     const bool nh_only = (P.tag_mask & ~2u) == 0u;
@@ -2232,9 +2265,41 @@ hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b,
     st.model = s.model;
     st.dp0 = s.dp0;
     st.dp1 = s.dp1;
+    st.scene_rec = s.scene_rec;
     auto kern = P.tag_mask == 2u ? k_particle_update<2u, false>
                                  : (P.svd_f64 ? k_particle_update<15u, true> : k_particle_update<15u, false>);
     launch_kernel(ev, kern, dim3(div_up_t(P.n, 256)), dim3(256), 0, stream, P, to_pbuf(b), st, pcap, fdt);
+    return hipGetLastError();
+}
+
+// For each run of equal scene ids among slots [slot0, slot0 + n): the by-id
+// records of the run's first particle into its scene's table entry.
+__global__ __launch_bounds__(256) void k_scene_records(const float4 *__restrict__ pos, const uint32_t *__restrict__ ids,
+                                                       uint32_t slot0, uint32_t n, PStatic st,
+                                                       float4 *__restrict__ scene_rec) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t scene = __float_as_uint(pos[slot0 + i].w) >> 1;
+    if (i > 0 && (__float_as_uint(pos[slot0 + i - 1].w) >> 1) == scene) return;
+    const uint32_t oid = ids[slot0 + i];
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    scene_rec[4 * (size_t)scene] = st.prop[oid];
+    scene_rec[4 * (size_t)scene + 1] = st.model[oid];
+    scene_rec[4 * (size_t)scene + 2] = st.dp0 ? st.dp0[oid] : zero;
+    scene_rec[4 * (size_t)scene + 3] = st.dp1 ? st.dp1[oid] : zero;
+}
+
+hipError_t launch_scene_records(const float4 *pos, const uint32_t *ids, uint32_t slot0, uint32_t n,
+                                const StaticBuffers &s, float4 *scene_rec, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    PStatic st;
+    st.prop = s.prop;
+    st.model = s.model;
+    st.dp0 = s.dp0;
+    st.dp1 = s.dp1;
+    st.scene_rec = nullptr;
+    hipLaunchKernelGGL(k_scene_records, dim3(div_up_t(n, 256)), dim3(256), 0, stream, pos, ids, slot0, n, st,
+                       scene_rec);
     return hipGetLastError();
 }
 
@@ -2246,6 +2311,7 @@ hipError_t launch_timestep_bound(const StepParams &P, const ParticleBuffers &b, 
     st.model = s.model;
     st.dp0 = s.dp0;
     st.dp1 = s.dp1;
+    st.scene_rec = s.scene_rec;
     hipLaunchKernelGGL(k_timestep_bound, dim3(min_t(div_up_t(P.n, 256), 4096)), dim3(256), 0, stream, P, to_pbuf(b),
                        st, pcap, out);
     return hipGetLastError();
@@ -2258,6 +2324,7 @@ hipError_t launch_timestep_bound_partials(const StepParams &P, const ParticleBuf
     st.model = s.model;
     st.dp0 = s.dp0;
     st.dp1 = s.dp1;
+    st.scene_rec = s.scene_rec;
     const uint32_t groups = div_up_t(P.n, TSB_CHUNK);
     if (groups > 0)
         hipLaunchKernelGGL(k_timestep_bound_partials, dim3(groups), dim3(256), 0, stream, P, to_pbuf(b), st, pcap,

@@ -49,6 +49,7 @@ struct ParticleBuffers {
 
 struct StaticBuffers {
     const float4 *prop, *model, *dp0, *dp1;
+    const float4 *scene_rec;  // batched: 4 records (prop, model, dp0, dp1) per scene id
 };
 
 struct ColliderBuffers {
@@ -88,6 +89,11 @@ hipError_t launch_g2p(const StepParams &P, const ParticleBuffers &b, uint32_t pc
 hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
                                   float *fdt, const PassEvents *ev, hipStream_t stream);
 hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, const PassEvents *ev, hipStream_t stream);
+// Per-scene record table: for each run of equal scene ids among storage slots
+// [slot0, slot0 + n) of a particle buffer, the by-id records of the run's first
+// particle (the table is read only while every scene has one record).
+hipError_t launch_scene_records(const float4 *pos, const uint32_t *ids, uint32_t slot0, uint32_t n,
+                                const StaticBuffers &s, float4 *scene_rec, hipStream_t stream);
 hipError_t launch_timestep_bound(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
                                  uint32_t *out, hipStream_t stream);
 // Two passes over partial minima, one per TSB_CHUNK particles, in `partials`

@@ -157,6 +157,14 @@ struct slosh_data {
     std::vector<HostRun> runs;
     uint64_t tag_count[5] = {0, 0, 0, 0, 0};  // particles per material tag (4: other)
     int uni_state = 0;                        // 0: no particles, 1: uniform (u_* valid), 2: mixed
+    // Batched data: per scene id, 0 no particle given, 1 one static record (its
+    // first, in scene_first), 2 several; scenes_mixed counts the 2s. While it is
+    // zero the kernels read each scene's record from scene_rec (filled on the
+    // device from the uploaded by-id records) instead of gathering by id.
+    std::vector<uint8_t> scene_state;
+    std::vector<float4> scene_first;
+    uint32_t scenes_mixed = 0;
+    float4 *scene_rec = nullptr;  // [4 * SLOSH_MAX_SCENES]: prop, model, dp0, dp1
     // Particle masses: 0 no particles, 1 every particle created enabled with mass
     // u_mass (bitwise) and not exposed to hooks or device views since, 2 otherwise.
     int mass_state = 0;
@@ -190,13 +198,15 @@ struct slosh_data {
     size_t mesh_nv = 0, mesh_ni = 0;
     bool has_trimesh = false;
     // Pinned host arenas that appends stage particles in, each reused once the
-    // uploads queued from it (marked by `done`) have completed.
+    // uploads queued from it (marked by `done`) have completed. Up to 32, so that
+    // a burst of admissions while substeps run does not wait for them: the uploads
+    // of an arena in use are queued behind those substeps.
     struct StageArena {
         uint8_t *ptr = nullptr;
         size_t bytes = 0;
         hipEvent_t done = nullptr;
     };
-    static constexpr int MAX_STAGE_ARENAS = 4;
+    static constexpr int MAX_STAGE_ARENAS = 32;
     StageArena stage[MAX_STAGE_ARENAS];
     int num_stage = 0;
     uint2 *mats = nullptr;
@@ -414,6 +424,34 @@ static void note_masses(slosh_data *d, const slosh_kinematics *k, size_t n) {
     }
 }
 
+// Per-scene record tracking (batched data): the static records of one particle
+// of scene `sc`, compared as note_records compares them.
+static void note_scene(slosh_data *d, uint32_t sc, const slosh_particle_model &m, const slosh_particle_properties &pr) {
+    if (d->scene_state.empty()) {
+        d->scene_state.assign(SLOSH_MAX_SCENES, 0);
+        d->scene_first.assign(4 * (size_t)SLOSH_MAX_SCENES, float4{});
+    }
+    if (d->scene_state[sc] == 2) return;
+    float4 r[4];
+    static_records(pr, m, &r[0], &r[1], &r[2], &r[3]);
+    float4 *f = &d->scene_first[4 * (size_t)sc];
+    if (d->scene_state[sc] == 0) {
+        for (int k = 0; k < 4; k++) f[k] = r[k];
+        d->scene_state[sc] = 1;
+    } else if (std::memcmp(&r[0], &f[0], 12) != 0 || std::memcmp(&r[1], &f[1], 16) != 0 ||
+               std::memcmp(&r[2], &f[2], 16) != 0 || std::memcmp(&r[3], &f[3], 8) != 0) {
+        d->scene_state[sc] = 2;
+        d->scenes_mixed++;
+    }
+}
+
+// A removed scene's id may be reused with other records.
+static void forget_scene(slosh_data *d, uint32_t sc) {
+    if (d->scene_state.empty()) return;
+    if (d->scene_state[sc] == 2) d->scenes_mixed--;
+    d->scene_state[sc] = 0;
+}
+
 static bool same_records(const slosh_particle_model &a, const slosh_particle_properties &pa,
                          const slosh_particle_model &b, const slosh_particle_properties &pb) {
     return std::memcmp(&a, &b, sizeof a) == 0 && std::memcmp(&pa, &pb, sizeof pa) == 0;
@@ -435,6 +473,11 @@ static void host_append(slosh_data *d, const slosh_particles *p) {
         bool uni = true;
         for (size_t k = i + 1; k < j && uni; k++)
             uni = same_records(p->models[k], p->properties[k], p->models[i], p->properties[i]);
+        if (d->batched) {
+            if (uni) note_scene(d, sc, p->models[i], p->properties[i]);
+            else
+                for (size_t k = i; k < j; k++) note_scene(d, sc, p->models[k], p->properties[k]);
+        }
         if (uni && r.count == 0) {
             r.recs_m.assign(1, p->models[i]);
             r.recs_p.assign(1, p->properties[i]);
@@ -645,6 +688,10 @@ static int upload_staging(slosh_data *d, const S &s, int b, size_t slot0, size_t
         HIP_TRY(hipMemcpyAsync(d->s_dp0 + id0, &s.dp0[0], 16 * n, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(d->s_dp1 + id0, &s.dp1[0], 16 * n, hipMemcpyHostToDevice, st));
     }
+    if (d->scene_rec) {
+        StaticBuffers sb{d->s_prop, d->s_model, d->s_dp0, d->s_dp1, nullptr};
+        HIP_TRY(launch_scene_records(p.pos, p.ids, (uint32_t)slot0, (uint32_t)n, sb, d->scene_rec, st));
+    }
     if (wait) HIP_TRY(hipStreamSynchronize(st));
     return SLOSH_OK;
 }
@@ -708,6 +755,7 @@ static StepParams make_params(const slosh_data *d) {
     P.svd_f64 = d->svd_f64;
     P.tag_mask = d->tag_mask;
     P.uniform = d->uniform;
+    P.scene_uniform = (d->batched && d->scene_rec && d->scenes_mixed == 0 && !d->uniform) ? 1u : 0u;
     P.p2g_chunk = d->tuning.p2g_chunk;
     P.g2p_threads = d->tuning.g2p_threads;
     P.block_wgs = d->tuning.block_wgs;
@@ -738,6 +786,7 @@ static StaticBuffers sbufs(slosh_data *d) {
     s.model = d->s_model;
     s.dp0 = d->s_dp0;
     s.dp1 = d->s_dp1;
+    s.scene_rec = d->scene_rec;
     return s;
 }
 
@@ -993,6 +1042,7 @@ static void free_data(slosh_data *d) {
     dfree(d->mats);
     dfree(d->impulses);
     dfree(d->dead);
+    dfree(d->scene_rec);
     dfree(d->mesh_vtx);
     dfree(d->mesh_idx);
     dfree(d->dt_bound);
@@ -1105,6 +1155,7 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     d->has_sand = (any_sand || d->batched) ? 1u : 0u;
     d->has_force = any_force ? 1u : 0u;
     TRY(alloc_particle_buffers(d, d->pcap, d->has_sand != 0));
+    if (d->batched) TRY(dalloc(&d->scene_rec, 4 * (size_t)SLOSH_MAX_SCENES));
     host_append(d, particles);
     note_records(d, particles->models, particles->properties, particles->len, 1);
     note_masses(d, particles->kinematics, particles->len);
@@ -1964,6 +2015,7 @@ static int device_remove(slosh_data *d, hipStream_t st, uint32_t mode, const std
         if (mode == 0) {
             if (dead_host[r.scene]) {
                 unnote_run(d, r, 0, len);
+                forget_scene(d, r.scene);
             } else {
                 m += len;
                 kept.push_back(std::move(r));

@@ -205,3 +205,74 @@ def test_batched_key_range_is_reported(gpu_pipeline):
     # the resting scene is untouched by the other's escape
     pos = data.read_scene(1, st, positions_only=True)
     assert np.allclose(pos[:, :3], b.scenes[1].particles["pos4"][:, :3], atol=1e-6)
+
+
+def _mixed_record_scene():
+    """A batched-configuration scene whose particles do not share one static
+    record (every other particle gets another Young's modulus)."""
+    from slosh_amd import models as M
+    sc = scenes.stir(scenes.batched_scene(5, n_override=2500), 93)
+    sc.particles["models"][::2] = np.asarray(M.elastic_neo_hookean(5.0e4, 0.3), np.uint32)
+    return sc
+
+
+@pytest.mark.parametrize("steps", [1, 20])
+def test_batch_with_a_mixed_record_scene(gpu_pipeline, steps):
+    """Scenes of one record each read it from the per-scene table; a scene whose
+    particles differ makes every scene read its records by particle id. Both
+    against the oracle on each scene alone."""
+    pl, st = gpu_pipeline
+    scs = [scenes.stir(scenes.batched_scene(0, n_override=2000), 81), _mixed_record_scene(),
+           scenes.stir(scenes.batched_scene(2, n_override=3000), 83)]
+    b = scenes.Batch(scs, ids=[4, 9, 2])
+    data = P.MpmData.from_batch(pl, b)
+    for _ in range(steps):
+        pl.launch_step(data, st)
+    data.check_errors(st)
+    for sc, sid in zip(b.scenes, b.ids):
+        orc = O.Oracle(sc, svd_mode=1)
+        orc.step(steps)
+        errs = state_errors(data.read_scene(sid, st), (orc.positions(), orc.kinematics(), orc.def_grad()),
+                            sc.particles["pos4"])
+        print(f"\n[mixed-record batch, scene {sid} after {steps}] " +
+              " ".join(f"{k}={v:.3e}" for k, v in errs.items() if k != "enabled_diff"))
+        assert errs["enabled_diff"] == 0
+        for k in ("x", "v", "affine", "F"):
+            assert errs[k] <= tol_for(steps), (sid, k, errs[k])
+
+
+def test_scene_id_reused_with_another_material(gpu_pipeline):
+    """A removed scene's id taken by a scene of another material (its per-scene
+    record is rewritten), and a mixed-record scene appended and removed again
+    (by-id records while it runs, the table afterwards): every scene still
+    matches the oracle alone."""
+    pl, st = gpu_pipeline
+    s0 = scenes.stir(scenes.batched_scene(0, n_override=2000), 61)
+    s1 = scenes.stir(scenes.batched_scene(1, n_override=1500), 62)
+    b = scenes.Batch([s0, s1], ids=[0, 1])
+    data = P.MpmData.from_batch(pl, b, particle_capacity=b.n + 8000)
+    for _ in range(3):
+        pl.launch_step(data, st)
+    data.remove_scenes([1], st)
+    # Scene id 1 again, with the first scene of another material in the batch configuration.
+    tag1 = int(s1.particles["models"][0][0])
+    k = next(i for i in range(2, 40) if int(scenes.batched_scene(i, n_override=8).particles["models"][0][0]) != tag1)
+    s2 = scenes.stir(scenes.batched_scene(k, n_override=2500), 63)
+    data.append(s2.particles, scene_ids=np.full(s2.n, 1, np.uint32), stream=st)
+    mx = _mixed_record_scene()
+    data.append(mx.particles, scene_ids=np.full(mx.n, 6, np.uint32), stream=st)
+    for _ in range(4):
+        pl.launch_step(data, st)
+    data.remove_scenes([6], st)
+    for _ in range(4):
+        pl.launch_step(data, st)
+    data.check_errors(st)
+    for sc, sid, steps in ((s0, 0, 11), (s2, 1, 8)):
+        orc = O.Oracle(sc, svd_mode=1)
+        orc.step(steps)
+        errs = state_errors(data.read_scene(sid, st), (orc.positions(), orc.kinematics(), orc.def_grad()),
+                            sc.particles["pos4"])
+        print(f"\n[scene {sid} after {steps}] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()))
+        assert errs["enabled_diff"] == 0
+        for key in ("x", "v", "affine", "F"):
+            assert errs[key] <= tol_for(steps), (sid, key, errs[key])
