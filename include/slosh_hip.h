@@ -313,6 +313,13 @@ int slosh_read_block_scenes(slosh_data *data, slosh_stream stream, uint32_t *out
  * by `data` before it returns (the caller's arrays may be reused at once) and
  * leaves the uploads queued on `stream`, ahead of the substeps launched next. */
 int slosh_append_particles(slosh_data *data, slosh_stream stream, const slosh_particles *particles);
+/* No reference counterpart (serving): pins `bytes` of host memory as a ring that
+ * later appends stage in (200 bytes per particle), so that admitting a scene pins
+ * nothing; an append that does not fit the ring falls back to the arenas. An
+ * append takes the region after the previous one and waits only while the ring is
+ * full (for the oldest region's uploads). 0 releases the ring. Synchronizes with
+ * the uploads still reading the previous ring. */
+int slosh_reserve_staging(slosh_data *data, size_t bytes);
 int slosh_remove_particles(slosh_data *data, slosh_stream stream, size_t start, size_t end);
 /* Removes every particle of `scene_id` / of the listed scenes (batched mode; no
  * reference counterpart: scenes are this library's batching unit). */

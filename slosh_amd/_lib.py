@@ -24,7 +24,7 @@ EXPORTS = [
     "slosh_event_create", "slosh_event_destroy", "slosh_event_record", "slosh_event_elapsed_ms",
     "slosh_device_count", "slosh_last_error", "slosh_version", "slosh_graph_capture", "slosh_graph_launch",
     "slosh_graph_destroy", "slosh_set_tuning", "slosh_get_tuning", "slosh_autotune", "slosh_timestep_bound",
-    "slosh_adaptive_substeps", "slosh_timestep_bound_noalloc",
+    "slosh_adaptive_substeps", "slosh_timestep_bound_noalloc", "slosh_reserve_staging",
 ]
 
 
@@ -116,6 +116,7 @@ def load():
         "slosh_autotune": ([vp, vp, vp, u32, vp], i32),
         "slosh_timestep_bound": ([vp, vp, vp, vp], i32),
         "slosh_timestep_bound_noalloc": ([vp, vp, vp], i32),
+        "slosh_reserve_staging": ([vp, sz], i32),
         "slosh_adaptive_substeps": ([vp, vp, C.c_float, u32, u32, vp], i32),
         "slosh_scene_size": ([vp, u32], sz),
         "slosh_read_scene": ([vp, vp, u32, vp, vp, vp, sz, vp], i32),

@@ -87,6 +87,9 @@ struct StepParams {
     // Batched data whose every scene has one static record: read it from the
     // per-scene table (scene id from the position's w bits) instead of by id.
     uint32_t scene_uniform;
+    // Batched data whose every scene's particles were given with mass = enabled ?
+    // m_scene : +0 (and not exposed since): the mass comes from the scene table.
+    uint32_t scene_mass;
     // Every particle was created enabled with mass u_mass, and only the update has
     // disabled particles since (it zeroes their mass): mass = enabled ? u_mass : 0.
     uint32_t mass_uniform;

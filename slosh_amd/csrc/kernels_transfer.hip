@@ -1868,19 +1868,26 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
                 float aff[9];
                 float vgd;
                 g2p_particle(sh, P.h, P.inv_d, pos, enabled != 0u, &v, aff, &vgd);
-                // Uniform masses skip the 16-byte velocity / mass load (the velocity
-                // comes from the grid): only the update disables, and it zeroes the mass.
-                float mass = P.mass_uniform ? (enabled ? P.u_mass : 0.0f) : in.vm[p].w;
+                const uint32_t scene = wbits >> 1;
+                // Uniform masses (one for all, or one per scene) skip the 16-byte
+                // velocity / mass load (the velocity comes from the grid): only the
+                // update disables, and it zeroes the mass.
+                float mass = P.mass_uniform ? (enabled ? P.u_mass : 0.0f)
+                             : P.scene_mass  ? (enabled ? st.scene_rec[4 * (size_t)scene + 3].z : 0.0f)
+                                             : in.vm[p].w;
                 if (!EARLY) {
 #pragma unroll
                     for (int q = 0; q < 9; q++) F[q] = in.F[(size_t)q * pcap + p];
                     oid = in.ids[p];
                 }
-                const uint32_t scene = wbits >> 1;
                 float4 prop, model;
                 static_records(P, st, oid, scene, &prop, &model);
+                // Only Drucker-Prager particles read or write their plastic state
+                // (the other particles' state slots are never read).
+                const uint32_t tag = __float_as_uint(model.x);
+                const bool plastic = P.has_sand && (tag == 2u || tag == 3u);
                 float dps[3] = {0.0f, 0.0f, 0.0f};
-                if (P.has_sand) {
+                if (plastic) {
 #pragma unroll
                     for (int q = 0; q < 3; q++) dps[q] = in.dps[(size_t)q * pcap + p];
                 }
@@ -1894,7 +1901,7 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
                 }
                 out.vgd[slot] = vgd;
                 out.ids[slot] = oid;
-                if (P.has_sand) {
+                if (plastic) {
 #pragma unroll
                     for (int q = 0; q < 3; q++) out.dps[(size_t)q * pcap + slot] = dps[q];
                 }
@@ -2289,7 +2296,19 @@ __global__ __launch_bounds__(256) void k_scene_records(const float4 *__restrict_
     scene_rec[4 * (size_t)scene + 3] = st.dp1 ? st.dp1[oid] : zero;
 }
 
-hipError_t launch_scene_records(const float4 *pos, const uint32_t *ids, uint32_t slot0, uint32_t n,
+// Then each scene's mass, the largest particle mass of its uploaded particles
+// (the enabled ones share it, the disabled ones have +0), into the unused third
+// component of its plastic record (zeroed by k_scene_records). As unsigned bits,
+// non-negative floats order as their values.
+__global__ __launch_bounds__(256) void k_scene_mass(const float4 *__restrict__ pos, const float4 *__restrict__ vm,
+                                                    uint32_t slot0, uint32_t n, float4 *__restrict__ scene_rec) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t scene = __float_as_uint(pos[slot0 + i].w) >> 1;
+    atomicMax((uint32_t *)&scene_rec[4 * (size_t)scene + 3].z, __float_as_uint(vm[slot0 + i].w));
+}
+
+hipError_t launch_scene_records(const float4 *pos, const float4 *vm, const uint32_t *ids, uint32_t slot0, uint32_t n,
                                 const StaticBuffers &s, float4 *scene_rec, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     PStatic st;
@@ -2300,6 +2319,7 @@ hipError_t launch_scene_records(const float4 *pos, const uint32_t *ids, uint32_t
     st.scene_rec = nullptr;
     hipLaunchKernelGGL(k_scene_records, dim3(div_up_t(n, 256)), dim3(256), 0, stream, pos, ids, slot0, n, st,
                        scene_rec);
+    hipLaunchKernelGGL(k_scene_mass, dim3(div_up_t(n, 256)), dim3(256), 0, stream, pos, vm, slot0, n, scene_rec);
     return hipGetLastError();
 }
 

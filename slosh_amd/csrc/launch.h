@@ -91,8 +91,10 @@ hipError_t launch_particle_update(const StepParams &P, const ParticleBuffers &b,
 hipError_t launch_rigid_update(const StepParams &P, const BodyBuffers &bb, const PassEvents *ev, hipStream_t stream);
 // Per-scene record table: for each run of equal scene ids among storage slots
 // [slot0, slot0 + n) of a particle buffer, the by-id records of the run's first
-// particle (the table is read only while every scene has one record).
-hipError_t launch_scene_records(const float4 *pos, const uint32_t *ids, uint32_t slot0, uint32_t n,
+// particle (read only while every scene has one record), then each scene's mass
+// (the largest of those particles' masses) in the plastic record's z (read only
+// while every scene's particles have mass = enabled ? its mass : +0).
+hipError_t launch_scene_records(const float4 *pos, const float4 *vm, const uint32_t *ids, uint32_t slot0, uint32_t n,
                                 const StaticBuffers &s, float4 *scene_rec, hipStream_t stream);
 hipError_t launch_timestep_bound(const StepParams &P, const ParticleBuffers &b, const StaticBuffers &s, uint32_t pcap,
                                  uint32_t *out, hipStream_t stream);

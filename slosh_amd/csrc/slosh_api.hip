@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -164,6 +165,14 @@ struct slosh_data {
     std::vector<uint8_t> scene_state;
     std::vector<float4> scene_first;
     uint32_t scenes_mixed = 0;
+    // Per scene id, as mass_state per scene: 0 no particle given, 1 every particle
+    // given with mass = enabled ? scene_mass_bits : +0 in one upload, 2 otherwise;
+    // scene_masses_mixed counts the 2s. scene_masses_off once masses were
+    // rewritten by the caller.
+    std::vector<uint8_t> scene_mass_state;
+    std::vector<uint32_t> scene_mass_bits;
+    uint32_t scene_masses_mixed = 0;
+    bool scene_masses_off = false;
     float4 *scene_rec = nullptr;  // [4 * SLOSH_MAX_SCENES]: prop, model, dp0, dp1
     // Particle masses: 0 no particles, 1 every particle created enabled with mass
     // u_mass (bitwise) and not exposed to hooks or device views since, 2 otherwise.
@@ -209,6 +218,18 @@ struct slosh_data {
     static constexpr int MAX_STAGE_ARENAS = 32;
     StageArena stage[MAX_STAGE_ARENAS];
     int num_stage = 0;
+    // A pinned ring reserved ahead of time (slosh_reserve_staging): an append that
+    // fits takes the region after the last one, so admissions pin nothing. Regions
+    // are freed in order, once the uploads queued from them (marked by `done`) have
+    // completed.
+    struct RingRegion {
+        size_t off, len;
+        hipEvent_t done;
+    };
+    uint8_t *ring = nullptr;
+    size_t ring_bytes = 0, ring_head = 0;
+    std::deque<RingRegion> ring_used;
+    std::vector<hipEvent_t> ring_events;  // idle events for new regions
     uint2 *mats = nullptr;
     int *impulses = nullptr;
     // Captured substep pairs, one per starting parity, each with the step
@@ -445,11 +466,60 @@ static void note_scene(slosh_data *d, uint32_t sc, const slosh_particle_model &m
     }
 }
 
+// Per-scene mass tracking of one upload (batched data). States: 0 no particle
+// given, 1 only disabled particles with mass +0, 3 also enabled ones, all with
+// mass scene_mass_bits, 2 anything else. A scene given particles by a second
+// upload counts as 2: the device takes a scene's mass from its latest upload.
+static void scene_mass_mixed(slosh_data *d, uint32_t sc) {
+    if (d->scene_mass_state[sc] != 2) {
+        d->scene_mass_state[sc] = 2;
+        d->scene_masses_mixed++;
+    }
+}
+
+static void note_scene_masses(slosh_data *d, const slosh_particles *p) {
+    if (d->scene_mass_state.empty()) {
+        d->scene_mass_state.assign(SLOSH_MAX_SCENES, 0);
+        d->scene_mass_bits.assign(SLOSH_MAX_SCENES, 0u);
+    }
+    std::vector<uint32_t> mine;  // the scenes this upload gives particles to
+    uint32_t last = SLOSH_NONE;
+    for (size_t i = 0; i < p->len; i++) {
+        const uint32_t sc = p->scene_ids[i];
+        if (sc != last) {
+            last = sc;
+            if (std::find(mine.begin(), mine.end(), sc) == mine.end()) {
+                mine.push_back(sc);
+                if (d->scene_mass_state[sc] != 0) scene_mass_mixed(d, sc);
+            }
+        }
+        uint8_t &st = d->scene_mass_state[sc];
+        if (st == 2) continue;
+        const slosh_kinematics &k = p->kinematics[i];
+        const uint32_t bits = f2u(k.mass);
+        if (!k.enabled) {
+            if (bits != 0u) scene_mass_mixed(d, sc);
+            else if (st == 0) st = 1;
+        } else if (st == 3) {
+            if (bits != d->scene_mass_bits[sc]) scene_mass_mixed(d, sc);
+        } else {
+            d->scene_mass_bits[sc] = bits;
+            st = 3;
+        }
+    }
+}
+
 // A removed scene's id may be reused with other records.
 static void forget_scene(slosh_data *d, uint32_t sc) {
-    if (d->scene_state.empty()) return;
-    if (d->scene_state[sc] == 2) d->scenes_mixed--;
-    d->scene_state[sc] = 0;
+    if (!d->scene_state.empty()) {
+        if (d->scene_state[sc] == 2) d->scenes_mixed--;
+        d->scene_state[sc] = 0;
+    }
+    if (!d->scene_mass_state.empty()) {
+        if (d->scene_mass_state[sc] == 2) d->scene_masses_mixed--;
+        d->scene_mass_state[sc] = 0;
+        d->scene_mass_bits[sc] = 0u;
+    }
 }
 
 static bool same_records(const slosh_particle_model &a, const slosh_particle_properties &pa,
@@ -616,12 +686,143 @@ static void stage_range(const slosh_particles *p, uint32_t id_base, S &s, size_t
 // particle into ~20 arrays (~40 ns per particle on one core), so large inputs
 // are split over up to 8 host threads; a range whose thread cannot be started
 // runs on the caller's.
+// Host threads for a pass over n particles (one below 2^17 particles).
+static size_t host_threads(size_t n) {
+    const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency() / 2);
+    return n < ((size_t)1 << 17) ? 1 : std::min<size_t>(std::min<size_t>(hw, 8), n >> 16);
+}
+
+// One pass over an append's particles (in parallel for large ones): whether they
+// all belong to the first one's scene and share its records, and how their
+// masses relate (enabled ones' bits all equal, disabled ones' all +0).
+struct AppendSummary {
+    bool one_scene = true, one_record = true;
+    bool has_enabled = false, enabled_equal = true, disabled_nonzero = false;
+    uint32_t mass_bits = 0u;
+};
+
+static AppendSummary summarize_range(const slosh_particles *p, size_t lo, size_t hi) {
+    AppendSummary a;
+    const uint32_t sc0 = p->scene_ids ? p->scene_ids[0] : 0u;
+    for (size_t i = lo; i < hi; i++) {
+        if (p->scene_ids && p->scene_ids[i] != sc0) a.one_scene = false;
+        if (a.one_record && !same_records(p->models[i], p->properties[i], p->models[0], p->properties[0]))
+            a.one_record = false;
+        const slosh_kinematics &k = p->kinematics[i];
+        const uint32_t bits = f2u(k.mass);
+        if (!k.enabled) {
+            if (bits != 0u) a.disabled_nonzero = true;
+        } else if (!a.has_enabled) {
+            a.has_enabled = true;
+            a.mass_bits = bits;
+        } else if (bits != a.mass_bits) {
+            a.enabled_equal = false;
+        }
+    }
+    return a;
+}
+
+static AppendSummary summarize(const slosh_particles *p) {
+    const size_t n = p->len, T = host_threads(n);
+    std::vector<AppendSummary> part(T);
+    std::vector<std::thread> workers;
+    for (size_t t = 1; t < T; t++) {
+        try {
+            workers.emplace_back([&, t]() { part[t] = summarize_range(p, n * t / T, n * (t + 1) / T); });
+        } catch (...) {
+            part[t] = summarize_range(p, n * t / T, n * (t + 1) / T);
+        }
+    }
+    part[0] = summarize_range(p, 0, n / T);
+    for (auto &w : workers) w.join();
+    AppendSummary a;
+    for (const AppendSummary &b : part) {
+        a.one_scene = a.one_scene && b.one_scene;
+        a.one_record = a.one_record && b.one_record;
+        a.disabled_nonzero = a.disabled_nonzero || b.disabled_nonzero;
+        a.enabled_equal = a.enabled_equal && b.enabled_equal;
+        if (b.has_enabled) {
+            if (!a.has_enabled) {
+                a.has_enabled = true;
+                a.mass_bits = b.mass_bits;
+            } else if (b.mass_bits != a.mass_bits) {
+                a.enabled_equal = false;
+            }
+        }
+    }
+    return a;
+}
+
+// host_append + note_records(+1) + note_masses (+ note_scene_masses) for one
+// append: for the common case (one scene, one record) from the summary, with
+// the same results as the per-particle passes, which run otherwise.
+static void note_append(slosh_data *d, const slosh_particles *p) {
+    if (p->len == 0) return;
+    const AppendSummary a = summarize(p);
+    const uint32_t sc = p->scene_ids ? p->scene_ids[0] : 0u;
+    HostRun *last = d->runs.empty() ? nullptr : &d->runs.back();
+    const bool joins = last && last->scene == sc && last->count > 0;
+    const bool fast = a.one_scene && a.one_record &&
+                      (!joins || (last->uniform && same_records(last->recs_m[0], last->recs_p[0], p->models[0],
+                                                                p->properties[0])));
+    if (!fast) {
+        host_append(d, p);
+        note_records(d, p->models, p->properties, p->len, 1);
+        note_masses(d, p->kinematics, p->len);
+        if (d->batched) note_scene_masses(d, p);
+        return;
+    }
+    // host_append of one uniform range
+    if (!last || last->scene != sc) {
+        d->runs.emplace_back();
+        d->runs.back().scene = sc;
+    }
+    HostRun &r = d->runs.back();
+    if (r.count == 0) {
+        r.recs_m.assign(1, p->models[0]);
+        r.recs_p.assign(1, p->properties[0]);
+        r.uniform = true;
+    }
+    if (d->batched) note_scene(d, sc, p->models[0], p->properties[0]);
+    r.count += p->len;
+    // note_records: every particle adds to its tag's count; the records are the first one's
+    d->tag_count[tag_slot(p->models[0])] += (uint64_t)p->len - 1u;
+    note_records(d, p->models, p->properties, 1, 1);
+    // note_masses
+    if (d->mass_state != 2) {
+        if (a.disabled_nonzero || !a.enabled_equal) {
+            d->mass_state = 2;
+        } else if (a.has_enabled) {
+            if (d->mass_state == 0) {
+                d->u_mass = u2f(a.mass_bits);
+                d->mass_state = 1;
+            } else if (a.mass_bits != f2u(d->u_mass)) {
+                d->mass_state = 2;
+            }
+        }
+    }
+    // note_scene_masses
+    if (d->batched) {
+        if (d->scene_mass_state.empty()) {
+            d->scene_mass_state.assign(SLOSH_MAX_SCENES, 0);
+            d->scene_mass_bits.assign(SLOSH_MAX_SCENES, 0u);
+        }
+        if (d->scene_mass_state[sc] != 0 || a.disabled_nonzero || !a.enabled_equal) {
+            scene_mass_mixed(d, sc);
+        } else if (a.has_enabled) {
+            d->scene_mass_bits[sc] = a.mass_bits;
+            d->scene_mass_state[sc] = 3;
+        } else {
+            d->scene_mass_state[sc] = 1;
+        }
+    }
+}
+
 template <typename S>
 static int stage_particles(const slosh_particles *p, uint32_t id_base, S &s, bool *any_force, bool *any_sand) {
     const size_t n = p->len;
     s.resize(n);
-    const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency() / 2);
-    const size_t T = n < ((size_t)1 << 17) ? 1 : std::min<size_t>(std::min<size_t>(hw, 8), n >> 16);
+    const size_t T = host_threads(n);
     std::vector<uint8_t> flags(3 * T, 0);
     std::vector<std::thread> workers;
     std::vector<size_t> inline_ranges;
@@ -690,7 +891,7 @@ static int upload_staging(slosh_data *d, const S &s, int b, size_t slot0, size_t
     }
     if (d->scene_rec) {
         StaticBuffers sb{d->s_prop, d->s_model, d->s_dp0, d->s_dp1, nullptr};
-        HIP_TRY(launch_scene_records(p.pos, p.ids, (uint32_t)slot0, (uint32_t)n, sb, d->scene_rec, st));
+        HIP_TRY(launch_scene_records(p.pos, p.vm, p.ids, (uint32_t)slot0, (uint32_t)n, sb, d->scene_rec, st));
     }
     if (wait) HIP_TRY(hipStreamSynchronize(st));
     return SLOSH_OK;
@@ -756,6 +957,10 @@ static StepParams make_params(const slosh_data *d) {
     P.tag_mask = d->tag_mask;
     P.uniform = d->uniform;
     P.scene_uniform = (d->batched && d->scene_rec && d->scenes_mixed == 0 && !d->uniform) ? 1u : 0u;
+    P.scene_mass = (d->batched && d->scene_rec && !d->masses_exposed && !d->scene_masses_off &&
+                    d->scene_masses_mixed == 0 && d->mass_state != 1)
+                       ? 1u
+                       : 0u;
     P.p2g_chunk = d->tuning.p2g_chunk;
     P.g2p_threads = d->tuning.g2p_threads;
     P.block_wgs = d->tuning.block_wgs;
@@ -1009,6 +1214,73 @@ int slosh_pipeline_create(const slosh_kernels *kernels, slosh_pipeline **out) {
 
 void slosh_pipeline_destroy(slosh_pipeline *p) { delete p; }
 
+// Waits for every region of the staging ring and frees it.
+static void release_ring(slosh_data *d) {
+    for (auto &r : d->ring_used) {
+        (void)hipEventSynchronize(r.done);
+        d->ring_events.push_back(r.done);
+    }
+    d->ring_used.clear();
+    if (d->ring) (void)hipHostFree(d->ring);
+    d->ring = nullptr;
+    d->ring_bytes = d->ring_head = 0;
+}
+
+// A region of `need` bytes of the staging ring at *off (false when the ring is
+// absent or smaller than `need`). Regions whose uploads completed are freed
+// first; while the ring is full, waits for the oldest.
+static bool ring_take(slosh_data *d, size_t need, size_t *off) {
+    need = (need + 255) & ~(size_t)255;
+    if (!d->ring || need >= d->ring_bytes) return false;
+    for (;;) {
+        while (!d->ring_used.empty()) {
+            const hipError_t q = hipEventQuery(d->ring_used.front().done);
+            (void)hipGetLastError();  // a not-ready query must not linger as the last error
+            if (q != hipSuccess) break;
+            d->ring_events.push_back(d->ring_used.front().done);
+            d->ring_used.pop_front();
+        }
+        if (d->ring_used.empty()) {
+            *off = 0;
+            return true;
+        }
+        // Used bytes run from the oldest region's start to ring_head (wrapping);
+        // a new region must end strictly before the oldest one starts.
+        const size_t tail = d->ring_used.front().off;
+        if (d->ring_head > tail) {
+            if (d->ring_bytes - d->ring_head >= need) {
+                *off = d->ring_head;
+                return true;
+            }
+            if (tail > need) {
+                *off = 0;
+                return true;
+            }
+        } else if (tail - d->ring_head > need) {
+            *off = d->ring_head;
+            return true;
+        }
+        (void)hipEventSynchronize(d->ring_used.front().done);
+    }
+}
+
+// Marks [off, off + need) of the ring as in use until the work queued on `st`
+// so far has completed.
+static int ring_commit(slosh_data *d, size_t off, size_t need, hipStream_t st) {
+    need = (need + 255) & ~(size_t)255;
+    hipEvent_t ev = nullptr;
+    if (!d->ring_events.empty()) {
+        ev = d->ring_events.back();
+        d->ring_events.pop_back();
+    } else {
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventRecord(ev, st));
+    d->ring_used.push_back({off, need, ev});
+    d->ring_head = off + need;
+    return SLOSH_OK;
+}
+
 static void free_data(slosh_data *d) {
     invalidate_graphs(d);
     for (slosh_graph *g : d->user_graphs) g->data = nullptr;  // their launches now fail cleanly
@@ -1046,6 +1318,9 @@ static void free_data(slosh_data *d) {
     dfree(d->mesh_vtx);
     dfree(d->mesh_idx);
     dfree(d->dt_bound);
+    release_ring(d);
+    for (hipEvent_t e : d->ring_events) (void)hipEventDestroy(e);
+    d->ring_events.clear();
     for (int i = 0; i < d->num_stage; i++) {
         if (d->stage[i].done) {
             (void)hipEventSynchronize(d->stage[i].done);
@@ -1096,25 +1371,52 @@ static int upload_bodies(slosh_data *d) {
 
 // Every particle's primary block, and its "+1" neighbours, must have exact keys
 // (block_associated_to_point, grid.slang:346-355, in the kernels' arithmetic).
-static int check_key_range(const slosh_particles *p, float h, uint32_t batched) {
+// The block of particle i (a coordinate of 1 << 24 for a non-finite or huge one).
+static void host_block_of(const slosh_particles *p, size_t i, float h, bool pow2, float inv, int b[3]) {
+    for (int c = 0; c < 3; c++) {
+        const float x = p->positions[4 * i + c];
+        const float q = pow2 ? x * inv : x / h;
+        const float a = std::nearbyint(q) - 1.0f;  // round half to even (default rounding mode)
+        const float f = std::floor(a / 4.0f);
+        b[c] = std::isfinite(f) && std::fabs(f) < 1.0e6f ? (int)f : (1 << 24);
+    }
+}
+
+// The first particle of [lo, hi) outside the block-key range, or hi.
+static size_t first_out_of_range(const slosh_particles *p, size_t lo, size_t hi, float h, uint32_t batched) {
     const bool pow2 = (f2u(h) & 0x807fffffu) == 0u && f2u(h) - 0x00800000u < 0x7e000000u;
     const float inv = 1.0f / h;
-    for (size_t i = 0; i < p->len; i++) {
+    for (size_t i = lo; i < hi; i++) {
         int b[3];
-        for (int c = 0; c < 3; c++) {
-            const float x = p->positions[4 * i + c];
-            const float q = pow2 ? x * inv : x / h;
-            const float a = std::nearbyint(q) - 1.0f;  // round half to even (default rounding mode)
-            const float f = std::floor(a / 4.0f);
-            b[c] = std::isfinite(f) && std::fabs(f) < 1.0e6f ? (int)f : (1 << 24);
-        }
-        if (!block_in_key_range(batched, b[0], b[1], b[2]))
-            return fail(SLOSH_E_KEY_RANGE, "particle " + std::to_string(i) + " lies in block (" + std::to_string(b[0]) +
-                                               ", " + std::to_string(b[1]) + ", " + std::to_string(b[2]) +
-                                               ") outside the block-key range" +
-                                               (batched ? " (batched: [-16, 46] per axis)" : ""));
+        host_block_of(p, i, h, pow2, inv, b);
+        if (!block_in_key_range(batched, b[0], b[1], b[2])) return i;
     }
-    return SLOSH_OK;
+    return hi;
+}
+
+static int check_key_range(const slosh_particles *p, float h, uint32_t batched) {
+    const size_t n = p->len, T = host_threads(n);
+    std::vector<size_t> bad(T, n);
+    std::vector<std::thread> workers;
+    for (size_t t = 1; t < T; t++) {
+        try {
+            workers.emplace_back([&, t]() { bad[t] = first_out_of_range(p, n * t / T, n * (t + 1) / T, h, batched); });
+        } catch (...) {
+            bad[t] = first_out_of_range(p, n * t / T, n * (t + 1) / T, h, batched);
+        }
+    }
+    bad[0] = first_out_of_range(p, 0, n / T, h, batched);
+    for (auto &w : workers) w.join();
+    size_t i = n;
+    for (size_t t = 0; t < T && i == n; t++)
+        if (bad[t] < n * (t + 1) / T) i = bad[t];
+    if (i == n) return SLOSH_OK;
+    int b[3];
+    const bool pow2 = (f2u(h) & 0x807fffffu) == 0u && f2u(h) - 0x00800000u < 0x7e000000u;
+    host_block_of(p, i, h, pow2, 1.0f / h, b);
+    return fail(SLOSH_E_KEY_RANGE, "particle " + std::to_string(i) + " lies in block (" + std::to_string(b[0]) + ", " +
+                                       std::to_string(b[1]) + ", " + std::to_string(b[2]) +
+                                       ") outside the block-key range" + (batched ? " (batched: [-16, 46] per axis)" : ""));
 }
 
 static int validate_particles(const slosh_particles *p) {
@@ -1156,9 +1458,7 @@ static int create_impl(slosh_pipeline *pipeline, const slosh_sim_params *params,
     d->has_force = any_force ? 1u : 0u;
     TRY(alloc_particle_buffers(d, d->pcap, d->has_sand != 0));
     if (d->batched) TRY(dalloc(&d->scene_rec, 4 * (size_t)SLOSH_MAX_SCENES));
-    host_append(d, particles);
-    note_records(d, particles->models, particles->properties, particles->len, 1);
-    note_masses(d, particles->kinematics, particles->len);
+    note_append(d, particles);
     HIP_TRY(hipStreamCreateWithFlags(&d->capture_stream, hipStreamNonBlocking));
     TRY(upload_staging(d, s, 0, 0, 0, d->capture_stream));
     const size_t cap = d->cap;
@@ -1907,6 +2207,15 @@ static int pick_stage_arena(slosh_data *d, size_t need, slosh_data::StageArena *
     return SLOSH_OK;
 }
 
+int slosh_reserve_staging(slosh_data *d, size_t bytes) {
+    if (!d) return fail(SLOSH_E_INVALID, "null data");
+    release_ring(d);
+    if (bytes == 0) return SLOSH_OK;
+    HIP_TRY(hipHostMalloc((void **)&d->ring, bytes, hipHostMallocDefault));
+    d->ring_bytes = bytes;
+    return SLOSH_OK;
+}
+
 int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_particles *particles) {
     if (!d) return fail(SLOSH_E_INVALID, "null data");
     TRY(validate_particles(particles));
@@ -1917,8 +2226,8 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
     if (particles->len == 0) return SLOSH_OK;
     bool any_force = false, any_sand = false;
     bool need_sand = false;
-    for (size_t i = 0; i < particles->len && !need_sand; i++) need_sand = model_is_sand(particles->models[i]);
-    need_sand = need_sand && !d->has_sand;
+    for (size_t i = 0; i < particles->len && !need_sand && !d->has_sand; i++)
+        need_sand = model_is_sand(particles->models[i]);
     if (d->n + particles->len <= d->pcap && !need_sand) {
         // Into the free tail: stage in a pinned arena and do the host bookkeeping
         // while substeps already queued on `stream` run, then queue the copies on
@@ -1927,16 +2236,17 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
         // released); without graphs the host goes on at once.
         const size_t need = PinnedStaging::WORDS * 4 * particles->len;
         slosh_data::StageArena *arena = nullptr;
-        TRY(pick_stage_arena(d, need, &arena));
-        PinnedStaging s(arena->ptr, particles->len);
+        size_t ring_off = 0;
+        const bool in_ring = ring_take(d, need, &ring_off);
+        if (!in_ring) TRY(pick_stage_arena(d, need, &arena));
+        PinnedStaging s(in_ring ? d->ring + ring_off : arena->ptr, particles->len);
         TRY(stage_particles(particles, (uint32_t)d->n, s, &any_force, &any_sand));
-        host_append(d, particles);
-        note_records(d, particles->models, particles->properties, particles->len, 1);
-        note_masses(d, particles->kinematics, particles->len);
+        note_append(d, particles);
         if (d->graph[0] || d->graph[1] || d->graph[2] || d->graph[3]) HIP_TRY(hipStreamSynchronize(st));
         invalidate_graphs(d);
         TRY(upload_staging(d, s, d->cur_p, d->n, d->n, st, false));
-        HIP_TRY(hipEventRecord(arena->done, st));
+        if (in_ring) TRY(ring_commit(d, ring_off, need, st));
+        else HIP_TRY(hipEventRecord(arena->done, st));
         d->n += particles->len;
         if (any_force) d->has_force = 1;
         return SLOSH_OK;
@@ -1954,9 +2264,7 @@ int slosh_append_particles(slosh_data *d, slosh_stream stream, const slosh_parti
         any_force = any_force || f2;
         TRY(rebuild(d, st, (d->n + particles->len) * 3 / 2 + 1024, all, any_sand));
     }
-    host_append(d, particles);
-    note_records(d, particles->models, particles->properties, particles->len, 1);
-    note_masses(d, particles->kinematics, particles->len);
+    note_append(d, particles);
     if (any_force) d->has_force = 1;
     return SLOSH_OK;
 }
@@ -2328,6 +2636,7 @@ int slosh_write_kinematics(slosh_data *d, slosh_stream stream, const slosh_kinem
         d->mass_state = 0;
         note_masses(d, kin, n);
     }
+    d->scene_masses_off = true;  // the per-scene masses are not re-derived
     invalidate_graphs(d);
     return SLOSH_OK;
 }

@@ -377,6 +377,11 @@ class MpmData:
         st, keep = _particles_struct(particles, scene_ids)
         _lib.check(_lib.load().slosh_append_particles(self.handle, self._s(stream), C.byref(st)))
 
+    def reserve_staging(self, nbytes):
+        """Pins `nbytes` of host memory as the ring appends stage in (200 bytes per
+        particle); 0 releases it."""
+        _lib.check(_lib.load().slosh_reserve_staging(self.handle, int(nbytes)))
+
     def shift_remove(self, start, end, stream=None):
         _lib.check(_lib.load().slosh_remove_particles(self.handle, self._s(stream), int(start), int(end)))
 

@@ -41,6 +41,8 @@ BYTES_PER_PARTICLE = 2 * (16 + 16 + 36 + 36 + 4 + 4 + 12) + 2 * 12 + 2 * 64 + 12
 # Device bytes per hash-grid slot: two tables (8 x 2), key 4, neighbours 64, cell
 # counts/starts 512, particle range 8, P2G tile 216 x 16, nodes 64 x 16.
 BYTES_PER_BLOCK = 16 + 4 + 64 + 512 + 8 + 216 * 16 + 64 * 16
+# Pinned host bytes an append stages one particle in.
+STAGING_BYTES_PER_PARTICLE = 200
 
 
 class Request:
@@ -83,6 +85,9 @@ class SceneServer:
     block_growth    blocks reserved per scene = block_growth x its initial estimate
                     (scenes spread while they run)
     result          "positions" (default) or "full" (positions, kinematics, F)
+    staging_bytes   pinned host ring reserved at construction for admissions to stage
+                    in (default: 200 bytes per particle of up to 16M particles), so
+                    that admitting a request pins no memory
     use_graphs      replay captured substep pairs instead of launching each substep.
                     Off by default: the pairs are re-captured after every change of
                     the batch's membership, and a batch of millions of particles
@@ -91,7 +96,8 @@ class SceneServer:
     """
 
     def __init__(self, pipeline, template, hbm_budget=200e9, max_particles=24 << 20, grid_capacity=1 << 20,
-                 grid_load=0.7, block_growth=3.0, max_chunk=64, result="positions", stream=None, use_graphs=False):
+                 grid_load=0.7, block_growth=3.0, max_chunk=64, result="positions", stream=None, use_graphs=False,
+                 staging_bytes=None):
         grid_capacity = S.pow2_at_least(grid_capacity)
         grid_bytes = grid_capacity * BYTES_PER_BLOCK
         if grid_bytes >= hbm_budget:
@@ -109,6 +115,10 @@ class SceneServer:
         self.data = P.MpmData(pipeline, template.params, _empty_particles(), template.bodies, template.materials,
                               template.cell_width, grid_capacity, scene_ids=np.zeros(0, np.uint32),
                               particle_capacity=self.particle_capacity)
+        if staging_bytes is None:
+            staging_bytes = STAGING_BYTES_PER_PARTICLE * min(self.particle_capacity, 16 << 20)
+        if staging_bytes:
+            self.data.reserve_staging(staging_bytes)
         self.cell_width = template.cell_width
         self.free_ids = collections.deque(range(MAX_SCENES))
         self.queue = collections.deque()
