@@ -1798,12 +1798,20 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
     init.mcount = ns.mcount;
     init.blk_cnt = nullptr;
     init.cell_cnt = nullptr;
+    // A block with stayers inserts its 8 blocks into the next table while the
+    // workgroup works on its next block (threads 0-7, key in `pk`): the first
+    // probe's load is issued with the next block's halo loads.
+    uint32_t pk = SLOSH_NONE;
     for (uint32_t k = blockIdx.x; k < B; k += gridDim.x) {
+        uint32_t probe = SLOSH_NONE;
+        if (pk != SLOSH_NONE) probe = nt.keys[hash_key(pk) & (P.cap - 1u)];
         const uint32_t hid = M.blist[k];
         const BlockRange br = block_range<FAST>(M, hid);
         const uint32_t cnt = br.total;
         if (cnt == 0) {
             if (threadIdx.x == 0) ns.stay[hid] = 0u;
+            if (pk != SLOSH_NONE && probe != pk) hash_insert(nt.keys, P.cap, pk, nt.counters, init);
+            pk = SLOSH_NONE;
             continue;
         }
         const uint32_t s0 = blk_start[hid];
@@ -1818,6 +1826,8 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
             n_stay = 0u;
         }
         __syncthreads();
+        if (pk != SLOSH_NONE && probe != pk) hash_insert(nt.keys, P.cap, pk, nt.counters, init);
+        pk = SLOSH_NONE;
         // The block's entries (a candidate is a member only if it is still in the
         // block), then, only if some of its movers overflowed their list, the
         // overflow list, whose entries of other blocks are skipped. Per member:
@@ -1951,16 +1961,14 @@ __global__ __launch_bounds__(NT, F64 ? (MODELS == 3u ? 4 : 3) : 1) void k_g2p_up
                 insert_blocks(nt.keys, P.cap, P.batched, next_key, heads, nt.counters, init);
             }
         }
+        // The barrier also ends this block's reads of `sh`, `n_out` and `n_stay`.
         const bool any_stays = __syncthreads_or(stays);
         if (threadIdx.x == 0) ns.stay[hid] = n_stay;
-        if (nt.prime && any_stays && threadIdx.x < 8) {
-            hash_insert(nt.keys, P.cap,
-                        pack_key(P.batched, scene, bx + BLOCK_ASSOC[threadIdx.x][0], by + BLOCK_ASSOC[threadIdx.x][1],
-                                 bz + BLOCK_ASSOC[threadIdx.x][2]),
-                        nt.counters, init);
-        }
-        __syncthreads();
+        if (nt.prime && any_stays && threadIdx.x < 8)
+            pk = pack_key(P.batched, scene, bx + BLOCK_ASSOC[threadIdx.x][0], by + BLOCK_ASSOC[threadIdx.x][1],
+                          bz + BLOCK_ASSOC[threadIdx.x][2]);
     }
+    if (pk != SLOSH_NONE) hash_insert(nt.keys, P.cap, pk, nt.counters, init);
 }
 
 // WgTimestepBounds (timestep_bound.slang:18-92): per enabled particle, the
