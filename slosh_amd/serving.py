@@ -130,6 +130,8 @@ class SceneServer:
         self.particle_substeps = 0
         self.chunks = 0
         self.seconds = {"admit": 0.0, "run": 0.0, "retire": 0.0, "idle": 0.0}
+        # Measured GPU seconds per particle-substep of a chunk (0 until the first chunk).
+        self.sec_per_ps = 0.0
 
     # -- admission / retirement -------------------------------------------------
     def submit(self, req):
@@ -140,11 +142,15 @@ class SceneServer:
     def _reservation(self, req):
         return int(np.ceil(self.block_growth * req.blocks))
 
-    def admit(self, now):
-        """Admits queued requests (FIFO) while particles, grid slots and scene ids allow."""
+    def admit(self, now, budget=None):
+        """Admits queued requests (FIFO) while particles, grid slots and scene ids
+        allow, and, with a time budget (seconds), only while the host time spent
+        admitting stays within it (at least one request per call)."""
         t0 = time.perf_counter()
         admitted = []
         while self.queue and self.free_ids:
+            if budget is not None and admitted and time.perf_counter() - t0 >= budget:
+                break
             req = self.queue[0]
             blocks = self._reservation(req)
             if self.in_flight + req.n > self.particle_capacity:
@@ -195,6 +201,7 @@ class SceneServer:
         batch (no further than substep count `until` if given) without waiting;
         returns the token finish_chunk takes."""
         t0 = time.perf_counter()
+        self._chunk_t0 = t0
         k = min(min(left for _, left in self.running.values()), self.max_chunk)
         if until is not None and until > self.substeps_run:
             k = min(k, int(until) - self.substeps_run)
@@ -211,6 +218,8 @@ class SceneServer:
         t0 = time.perf_counter()
         k, entries, in_flight = token
         self.stream.synchronize()
+        if k * in_flight > 0:
+            self.sec_per_ps = (time.perf_counter() - self._chunk_t0) / (k * in_flight)
         self.data.check_errors(self.stream)
         for entry in entries:
             entry[1] -= k
@@ -237,7 +246,11 @@ class SceneServer:
             now = clock()
             while pending and pending[0].arrival <= now:
                 self.submit(pending.popleft())
-            self.admit(now)
+            # On the wall clock, queued requests are admitted while a chunk runs
+            # (below); here only when nothing runs, and then only for a short
+            # while, so that the GPU starts on the first of a burst at once.
+            if not realtime or not self.running:
+                self.admit(now, budget=0.005 if realtime else None)
             if not self.running:
                 if pending and realtime:
                     t1 = time.perf_counter()
@@ -253,11 +266,15 @@ class SceneServer:
             if realtime:
                 # While the GPU runs the chunk, stage the requests that arrived
                 # meanwhile (their particles go to free slots the chunk does not
-                # touch; they join from the next chunk).
+                # touch; they join from the next chunk), for about as long as the
+                # chunk runs: a burst is admitted over several chunks, each larger
+                # than the last, instead of while the GPU waits.
                 now = clock()
                 while pending and pending[0].arrival <= now:
                     self.submit(pending.popleft())
-                self.admit(now)
+                k, _, in_flight = token
+                budget = 0.9 * self.sec_per_ps * k * in_flight if self.sec_per_ps > 0.0 else 0.005
+                self.admit(now, budget=budget)
             self.finish_chunk(token)
             finished += self.retire(clock)
         return finished
