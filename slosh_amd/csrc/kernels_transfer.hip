@@ -1254,9 +1254,10 @@ __device__ inline uint32_t key_at(float h, uint32_t batched, float4 x) {
 // the other, in 9 rounds in which the 64 lanes of a wave touch 64 distinct nodes
 // (plain read-modify-write; no LDS or global atomics on floats). Each cell's
 // summation order follows the order its particles were grouped in.
-// Cells hold ~8 particles, so lanes read particles ~8 apart: pad one word every
-// 8 to spread them over the LDS banks.
-__device__ inline uint32_t p2g_slot(uint32_t q) { return q + (q >> 3); }
+// Cells hold ~8 particles, so lanes read particles ~8 apart: one padding element
+// every 16 spreads 16 consecutive cells' first particles over the 16 LDS bank
+// groups of 16-byte elements.
+__device__ inline uint32_t p2g_slot(uint32_t q) { return q + (q >> 4); }
 
 // Next substep's hash table, reset here and filled with primary blocks by the
 // fused G2P / particle-update kernel (replaces k_reset_table + k_touch_primary
@@ -1313,20 +1314,26 @@ __device__ void p2g_block_scan(const Members &M, uint32_t *__restrict__ blk_star
     }
 }
 
-// Global index of LDS tile node j (x + 6 y + 36 z) in the region layout.
-__device__ inline uint32_t tile_global_of_lds(uint32_t j) {
-    const uint32_t x = j % 6u, y = (j / 6u) % 6u, z = j / 36u;
-    return tile_index(x, y, z);
+// LDS element of P2G tile node (x, y, z), 16-byte elements. Within a z plane,
+// x < 4 maps to x + 4 y (0..23) and x >= 4 to x + 4 y + 24 (28..49), one to one;
+// over any 4x4 patch of a plane (the 16 lanes of one merge access, at offsets
+// 0..2 in x and y) the element index mod 16, its LDS bank group, takes 16
+// different values (the patch's columns differ mod 4, and each column's 4 rows
+// cover the residues of that class).
+#define P2G_TILE_LDS 300
+__device__ inline uint32_t p2g_tile_lds(uint32_t x, uint32_t y, uint32_t z) {
+    return x + 4u * y + (x >= 4u ? 24u : 0u) + 50u * z;
 }
 
-// 40,848 bytes at 512 particles per chunk: four workgroups (12 waves) fit in a
+// 40,144 bytes at 512 particles per chunk: four workgroups (12 waves) fit in a
 // CU's 160 KiB of LDS, as many as the registers allow.
 template <int P2G_CHUNK>
 struct P2GShared {
-    static constexpr int LDS = P2G_CHUNK + P2G_CHUNK / 8;
-    float4 tile[TILE_NODES];
+    static constexpr int LDS = P2G_CHUNK + P2G_CHUNK / 16;
+    float4 tile[P2G_TILE_LDS];
     union {
-        float sp[16][LDS];    // m, A0[3], f[3], C[9], grouped by associated cell
+        // (m, A0), (f, C0), (C1..C4), (C5..C8) per member, grouped by associated cell
+        float4 sp4[4][LDS];
         uint32_t extra[192];  // this block's overflow movers of one round: read by
                               // p2g_chunk before it stages anything
     };
@@ -1344,7 +1351,6 @@ template <int P2G_CHUNK, typename SlotOf>
 __device__ inline void p2g_chunk(const StepParams &P, P2GShared<P2G_CHUNK> &S, uint32_t kn, SlotOf slot_of, const float4 *__restrict__ pos, const float4 *__restrict__ vm,
                                  const float *__restrict__ C, const float *__restrict__ fdt, uint32_t pcap,
                                  uint32_t block_key, float (&acc)[9][4], uint32_t &tot) {
-    constexpr int SP_M = 0, SP_A = 1, SP_F = 4, SP_C = 7;
     constexpr int R = (P2G_CHUNK + 191) / 192;
     const uint32_t kz = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const float h = P.h;
@@ -1422,30 +1428,27 @@ __device__ inline void p2g_chunk(const StepParams &P, P2GShared<P2G_CHUNK> &S, u
     for (int r = 0; r < R; r++) {
         if (pp[r] == SLOSH_NONE) continue;
         const uint32_t q = p2g_slot(S.lstart[cl[r]] + rk[r]);
-        S.sp[SP_M][q] = vv[r].w;
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            S.sp[SP_A + i][q] = a0[r][i];
-            S.sp[SP_F + i][q] = fo[r][i];
-        }
-#pragma unroll
-        for (int k = 0; k < 9; k++) S.sp[SP_C + k][q] = cc[r][k];
+        S.sp4[0][q] = make_float4(vv[r].w, a0[r][0], a0[r][1], a0[r][2]);
+        S.sp4[1][q] = make_float4(fo[r][0], fo[r][1], fo[r][2], cc[r][0]);
+        S.sp4[2][q] = make_float4(cc[r][1], cc[r][2], cc[r][3], cc[r][4]);
+        S.sp4[3][q] = make_float4(cc[r][5], cc[r][6], cc[r][7], cc[r][8]);
     }
     __syncthreads();
     for (uint32_t j = cs; j < cs + cnt; j++) {
         const uint32_t q = p2g_slot(j);
-        const float m = S.sp[SP_M][q];
-        const float wz = k_eval(S.sp[SP_F + 2][q] - (float)kz);
-        float c[9], wx[3], wy[3], wyz[3], az[3];
-        k_eval_all(S.sp[SP_F][q], wx);
-        k_eval_all(S.sp[SP_F + 1][q], wy);
-#pragma unroll
-        for (int k = 0; k < 9; k++) c[k] = S.sp[SP_C + k][q];
+        const float4 s0 = S.sp4[0][q], s1 = S.sp4[1][q], s2 = S.sp4[2][q], s3 = S.sp4[3][q];
+        const float m = s0.x;
+        const float a0s[3] = {s0.y, s0.z, s0.w};
+        const float wz = k_eval(s1.z - (float)kz);
+        float wx[3], wy[3], wyz[3], az[3];
+        const float c[9] = {s1.w, s2.x, s2.y, s2.z, s2.w, s3.x, s3.y, s3.z, s3.w};
+        k_eval_all(s1.x, wx);
+        k_eval_all(s1.y, wy);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             wyz[i] = wy[i] * wz;
             // A0 + C^T (0, 0, kz h): shared by the 9 nodes of the slab.
-            az[i] = S.sp[SP_A + i][q] + kzh * c[6 + i];
+            az[i] = a0s[i] + kzh * c[6 + i];
         }
 #pragma unroll
         for (int ky = 0; ky < 3; ky++) {
@@ -1497,7 +1500,7 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
     const uint32_t n_ovf = M.counters[CTR_OVERFLOW];
     const uint32_t kz = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
-    const uint32_t tbase = lx + 6 * ly + 36 * (lz + (int)kz);
+
     if (threadIdx.x < 64) S.lcnt[threadIdx.x] = 0u;
     for (uint32_t k = blockIdx.x - first; k < B; k += gridDim.x - first) {
         const uint32_t hid = M.blist[k];
@@ -1509,7 +1512,7 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
             continue;
         }
         __syncthreads();  // the previous block's tile has been written out
-        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 192) S.tile[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (uint32_t j = threadIdx.x; j < P2G_TILE_LDS; j += 192) S.tile[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const uint32_t block_key = keys[hid];
         float acc[9][4];
 #pragma unroll
@@ -1548,7 +1551,7 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
 #pragma unroll
                     for (int kx = 0; kx < 3; kx++) {
                         const int q = kx + 3 * ky;
-                        const uint32_t node = tbase + kx + 6 * ky;
+                        const uint32_t node = p2g_tile_lds(lx + kx, ly + ky, lz + kz);
                         float4 t = S.tile[node];
                         t.x += acc[q][0];
                         t.y += acc[q][1];
@@ -1562,8 +1565,10 @@ __global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const f
             }
             __syncthreads();
         }
-        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 192)
-            tiles[(size_t)hid * TILE_NODES + tile_global_of_lds(j)] = S.tile[j];
+        for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 192) {
+            const uint32_t x = j % 6u, y = (j / 6u) % 6u, z = j / 36u;
+            tiles[(size_t)hid * TILE_NODES + tile_index(x, y, z)] = S.tile[p2g_tile_lds(x, y, z)];
+        }
         if (kz == 0) cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane] = tot;
     }
 }
