@@ -36,11 +36,14 @@ MAX_SCENES = 16383
 
 # Device bytes per particle slot of a batched MpmData: two particle buffers
 # (pos 16, vel+mass 16, C 36, F 36, vel_grad_det 4, id 4, Drucker-Prager state 12),
-# force_dt 12 x 2, by-id records 64 x 2, sort scratch 12.
-BYTES_PER_PARTICLE = 2 * (16 + 16 + 36 + 36 + 4 + 4 + 12) + 2 * 12 + 2 * 64 + 12
-# Device bytes per hash-grid slot: two tables (8 x 2), key 4, neighbours 64, cell
-# counts/starts 512, particle range 8, P2G tile 216 x 16, nodes 64 x 16.
-BYTES_PER_BLOCK = 16 + 4 + 64 + 512 + 8 + 216 * 16 + 64 * 16
+# force_dt 12 x 2, by-id records 64 x 2, full-sort scratch 12 (slot, rank,
+# permutation), mover lists 16 (key, slot, overflow slot and block).
+BYTES_PER_PARTICLE = 2 * (16 + 16 + 36 + 36 + 4 + 4 + 12) + 2 * 12 + 2 * 64 + 12 + 16
+# Device bytes per hash-grid slot: per table (two) the key, block start, count,
+# stayers and movers (20 x 2), the block-list entry 4, neighbour table 64, cell
+# counts and starts 512, mover list 64 x 4, previous range 16, P2G tile 216 x 16,
+# nodes 64 x 16.
+BYTES_PER_BLOCK = 2 * 20 + 4 + 64 + 512 + 256 + 16 + 216 * 16 + 64 * 16
 # Pinned host bytes an append stages one particle in.
 STAGING_BYTES_PER_PARTICLE = 200
 

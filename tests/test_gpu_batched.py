@@ -333,3 +333,45 @@ def test_batch_scenes_of_different_densities(gpu_pipeline, steps):
         assert errs["enabled_diff"] == 0
         for k in ("x", "v", "affine", "F"):
             assert errs[k] <= max(tol_for(steps), 2.0 * floor[k]), (sid, k, errs[k], floor[k])
+
+
+def test_appends_staged_through_a_small_ring(gpu_pipeline):
+    """Appends while substeps are queued, staged in a 600 KB pinned ring (200 bytes
+    per particle: regions wrap around it and wait for the oldest uploads), one
+    append too large for the ring (staged in an arena instead), and one scene with
+    disabled particles: every scene matches the oracle run alone for its substeps."""
+    pl, st = gpu_pipeline
+    first = scenes.stir(scenes.batched_scene(0, n_override=1500), 51)
+    data = P.MpmData.from_batch(pl, scenes.Batch([first], ids=[0]), particle_capacity=30000)
+    data.reserve_staging(600_000)
+    runs = [(first, 0, 0)]  # scene, id, substep at which it joined
+    sizes = (1200, 2400, 1800, 4000, 2200, 1000)
+    step = 0
+    for k, n in enumerate(sizes):
+        for _ in range(2):
+            pl.launch_step(data, st)
+            step += 1
+        sc = scenes.stir(scenes.batched_scene(k + 1, n_override=n), 52 + k)
+        if k == 2:
+            kin = sc.particles["kin"]
+            kin["enabled"][::13] = 0
+            kin["mass"][::13] = 0.0
+        data.append(sc.particles, scene_ids=np.full(sc.n, k + 1, np.uint32), stream=st)
+        runs.append((sc, k + 1, step))
+    for _ in range(3):
+        pl.launch_step(data, st)
+        step += 1
+    data.check_errors(st)
+    for sc, sid, joined in runs:
+        steps = step - joined
+        orc = O.Oracle(sc, svd_mode=1)
+        orc.step(steps)
+        pos, kin, f = data.read_scene(sid, st)
+        assert np.array_equal(kin["mass"], orc.kinematics()["mass"])
+        errs = state_errors((pos, kin, f), (orc.positions(), orc.kinematics(), orc.def_grad()),
+                            sc.particles["pos4"])
+        print(f"\n[ring append, scene {sid} n={sc.n} after {steps}] " +
+              " ".join(f"{k}={v:.3e}" for k, v in errs.items() if k != "enabled_diff"))
+        assert errs["enabled_diff"] == 0
+        for key in ("x", "v", "affine", "F"):
+            assert errs[key] <= tol_for(steps), (sid, key, errs[key])
