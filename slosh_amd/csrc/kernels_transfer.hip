@@ -33,13 +33,6 @@ constexpr int ASSOC[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1},
 // ---------------------------------------------------------------------------
 // Quadratic B-spline (kernel.slang:160-174).
 
-__device__ inline float k_eval(float x) {
-    float xa = fabsf(x);
-    float p1 = 0.75f - xa * xa;
-    float p2 = 0.5f * (1.5f - xa) * (1.5f - xa);
-    return xa < 0.5f ? p1 : (xa < 1.5f ? p2 : 0.0f);
-}
-
 __device__ inline void k_eval_all(float x, float *w) {
     w[0] = 0.5f * (1.5f - x) * (1.5f - x);
     w[1] = 0.75f - (x - 1.0f) * (x - 1.0f);
@@ -1355,6 +1348,12 @@ __device__ inline void p2g_chunk(const StepParams &P, P2GShared<P2G_CHUNK> &S, u
     const uint32_t kz = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const float h = P.h;
     const float kzh = (float)kz * h;
+    // The slab's z weight is one piece of the quadratic B-spline of the offset f,
+    // the same for the whole wave: wz = a (f - b)^2 + c (no divergent branch).
+    const uint32_t kzu = __builtin_amdgcn_readfirstlane(kz);
+    const float wza = kzu == 1u ? -1.0f : 0.5f;
+    const float wzb = kzu == 0u ? 1.5f : (kzu == 1u ? 1.0f : 0.5f);
+    const float wzc = kzu == 1u ? 0.75f : 0.0f;
     __syncthreads();  // the previous chunk's readers are done; the cell counters are clear
     // All of this thread's loads are issued before any is consumed.
     uint32_t pp[R];
@@ -1439,7 +1438,8 @@ __device__ inline void p2g_chunk(const StepParams &P, P2GShared<P2G_CHUNK> &S, u
         const float4 s0 = S.sp4[0][q], s1 = S.sp4[1][q], s2 = S.sp4[2][q], s3 = S.sp4[3][q];
         const float m = s0.x;
         const float a0s[3] = {s0.y, s0.z, s0.w};
-        const float wz = k_eval(s1.z - (float)kz);
+        const float tz = s1.z - wzb;
+        const float wz = fmaf(wza * tz, tz, wzc);
         float wx[3], wy[3], wyz[3], az[3];
         const float c[9] = {s1.w, s2.x, s2.y, s2.z, s2.w, s3.x, s3.y, s3.z, s3.w};
         k_eval_all(s1.x, wx);
@@ -1472,8 +1472,10 @@ __device__ inline void p2g_chunk(const StepParams &P, P2GShared<P2G_CHUNK> &S, u
     if (kz == 0) S.lcnt[lane] = 0u;
 }
 
+// Chunks of up to 384 members fit in 128 VGPRs (4 waves/SIMD; their LDS allows 5
+// workgroups per CU); 512 needs ~156 (3 waves/SIMD, as its 40 KB of LDS allow).
 template <int P2G_CHUNK, bool FAST>
-__global__ __launch_bounds__(192) void k_p2g(StepParams P, NextTable nt, const float4 *__restrict__ pos,
+__global__ __launch_bounds__(192, P2G_CHUNK <= 384 ? 4 : 1) void k_p2g(StepParams P, NextTable nt, const float4 *__restrict__ pos,
                                              const float4 *__restrict__ vm, const float *__restrict__ C,
                                              const float *__restrict__ fdt, uint32_t pcap, Members M,
                                              const uint32_t *__restrict__ keys, float4 *__restrict__ tiles,
