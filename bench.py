@@ -6,8 +6,9 @@ neo-Hookean column, dx = 1/256 sparse hashed grid, dt = 2e-5, separating
 walls), seeded synthetic input resident in HBM before timing starts.
 
 Timed region: W warm-up substeps, then exactly K substeps captured once into a
-HIP graph and replayed, bracketed by stream synchronizations and HIP events on
-the launch stream. value = N * K / t.
+HIP graph and replayed (an odd K: K - 1 in the graph and one direct launch),
+bracketed by stream synchronizations and HIP events on the launch stream.
+value = N * K / t.
 
 Roofline: min(K, 20) more substeps launched directly behind one more replay of
 the timed graph (so that they run back to back), each pass timed by HIP events
@@ -121,11 +122,17 @@ def run_plain(args):
     pl.launch_steps_graph(data, max(args.warmup - 1, 0), st)
     st.synchronize()
     data.check_errors(st)
-    g_plain = P.Graph(pl, data, args.steps)
+    # A captured graph holds an even number of substeps (a pair returns the
+    # alternating buffers to where they started); an odd K adds one direct launch.
+    k_graph = args.steps - (args.steps & 1)
+    g_plain = P.Graph(pl, data, k_graph) if k_graph else None
     e0, e1 = P.Event(), P.Event()
     st.synchronize()
     e0.record(st)
-    g_plain.launch(st)
+    if g_plain is not None:
+        g_plain.launch(st)
+    if args.steps & 1:
+        pl.launch_step(data, st)
     e1.record(st)
     st.synchronize()
     ms = e0.elapsed_ms(e1)
@@ -140,7 +147,10 @@ def run_plain(args):
     ts = P.Timestamps()
     ts.reset()
     rsteps = max(2, min(args.steps, 20))
-    g_plain.launch(st)
+    if args.steps & 1:
+        pl.launch_step(data, st)  # back to the parity the graph was captured at
+    if g_plain is not None:
+        g_plain.launch(st)
     for _ in range(rsteps):
         pl.launch_step(data, st, timestamps=ts)
     per_pass = ts.read()
@@ -176,7 +186,8 @@ def run_plain(args):
             "cell_width": float(scene.cell_width),
             "dt": float(scene.params["dt"]),
             "active_blocks": int(n_blk),
-            "launch": f"hipGraph of {args.steps} substeps",
+            "launch": " + ".join(([f"hipGraph of {k_graph} substeps"] if k_graph else []) +
+                                 (["1 direct launch"] if args.steps & 1 else [])),
             "tuning": tuning,
         },
         "roofline": {
@@ -341,7 +352,8 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shape instead of autotuning it before timing")
     args = ap.parse_args()
-    args.steps += args.steps & 1  # the captured graph holds an even number of substeps
+    if args.steps < 1:
+        ap.error("--steps must be at least 1")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only (--gpus must be 1); sharding across "
                                    "GPUs is not implemented", "n_gpus": args.gpus}))
