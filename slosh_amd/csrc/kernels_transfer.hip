@@ -2,10 +2,10 @@
 // integration of one substep on gfx950.
 //
 //   k_p2g            one workgroup per active block: lane = associated cell,
-//                    each wave accumulates its particles' 27 stencil
-//                    contributions in registers and merges them into a 6x6x6
-//                    LDS tile (no atomics, LDS or global); the tile is written
-//                    once to HBM. Same per-(node, particle) arithmetic as
+//                    each wave accumulates its particles' contributions to one
+//                    z slab of their 27-node stencils in registers and sums them
+//                    into its own LDS slab tile (no atomics, LDS or global); the
+//                    6x6x6 tile, the slabs summed per node, is written once to HBM. Same per-(node, particle) arithmetic as
 //                    p2g_scatter_style.slang:84-148; only the summation order
 //                    differs (and is fixed for a given particle order).
 //   k_grid_update    one wave per block: node momentum = own tile + the halo
@@ -1243,10 +1243,12 @@ __device__ inline uint32_t key_at(float h, uint32_t batched, float4 x) {
 // stencil; each lane accumulates, over the particles of its cell in the chunk,
 // the contributions w_n (A0 + C^T (n h)) to the 9 nodes n = (kx, ky, kz) in 36
 // registers.
-// Phase 3: the waves merge their registers into the 6x6x6 LDS tile one after
-// the other, in 9 rounds in which the 64 lanes of a wave touch 64 distinct nodes
-// (plain read-modify-write; no LDS or global atomics on floats). Each cell's
-// summation order follows the order its particles were grouped in.
+// Phase 3: each wave sums its registers into its own slab tile (z planes kz to
+// kz + 3 of the 6x6x6 tile, in the LDS of the staging), in 9 rounds in which its
+// 64 lanes touch 64 distinct nodes (plain read-modify-write; no LDS or global
+// atomics on floats), all three waves at once; then each node of the tile is
+// the sum of the slabs that cover it, in wave order, written straight to HBM.
+// Each cell's summation order follows the order its particles were grouped in.
 // Cells hold ~8 particles, so lanes read particles ~8 apart: one padding element
 // every 16 spreads 16 consecutive cells' first particles over the 16 LDS bank
 // groups of 16-byte elements.
@@ -1307,32 +1309,35 @@ __device__ void p2g_block_scan(const Members &M, uint32_t *__restrict__ blk_star
     }
 }
 
-// LDS element of P2G tile node (x, y, z), 16-byte elements. Within a z plane,
+// LDS element of node (x, y, z) of a P2G slab tile (z = 0..3), 16-byte elements. Within a z plane,
 // x < 4 maps to x + 4 y (0..23) and x >= 4 to x + 4 y + 24 (28..49), one to one;
 // over any 4x4 patch of a plane (the 16 lanes of one merge access, at offsets
 // 0..2 in x and y) the element index mod 16, its LDS bank group, takes 16
 // different values (the patch's columns differ mod 4, and each column's 4 rows
 // cover the residues of that class).
-#define P2G_TILE_LDS 300
+#define P2G_SLAB_LDS 200
 __device__ inline uint32_t p2g_tile_lds(uint32_t x, uint32_t y, uint32_t z) {
     return x + 4u * y + (x >= 4u ? 24u : 0u) + 50u * z;
 }
 
-// 40,144 bytes at 512 particles per chunk: four workgroups (12 waves) fit in a
+// 35,088 bytes at 512 particles per chunk: four workgroups (12 waves) fit in a
 // CU's 160 KiB of LDS, as many as the registers allow.
 template <int P2G_CHUNK>
 struct P2GShared {
     static constexpr int LDS = P2G_CHUNK + P2G_CHUNK / 16;
-    float4 tile[P2G_TILE_LDS];
     union {
         // (m, A0), (f, C0), (C1..C4), (C5..C8) per member, grouped by associated cell
         float4 sp4[4][LDS];
+        // after the block's last chunk: each wave's sums over its slab (z planes kz..kz+3)
+        float4 slab[3][P2G_SLAB_LDS];
         uint32_t extra[192];  // this block's overflow movers of one round: read by
                               // p2g_chunk before it stages anything
     };
-    uint32_t lcnt[64], lstart[64];
+    uint32_t lcnt[64];
     uint32_t wsum[4];
 };
+static_assert(sizeof(float4) * 3 * P2G_SLAB_LDS <= sizeof(P2GShared<192>::sp4),
+              "the slab tiles take no LDS beyond the staging of the smallest chunk");
 static_assert(sizeof(P2GShared<512>) <= 163840 / 4, "P2G at 512 per chunk must fit four workgroups per CU");
 
 // Phase 1 and 2 for `kn` entries of the block's sequence; entry t's storage slot
@@ -1419,14 +1424,16 @@ __device__ inline void p2g_chunk(const StepParams &P, P2GShared<P2G_CHUNK> &S, u
         if ((int)lane >= d) x += y;
     }
     const uint32_t cs = x - cnt;
-    S.lstart[lane] = cs;
     if (kz == 0) tot += cnt;
-    __syncthreads();
+    // Each member's cell start from the lane of its cell (every wave holds all 64).
+    uint32_t cstart[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) cstart[r] = (uint32_t)__shfl((int)cs, pp[r] != SLOSH_NONE ? (int)cl[r] : 0);
     // Staging in cell order: a cell's members are consecutive.
 #pragma unroll
     for (int r = 0; r < R; r++) {
         if (pp[r] == SLOSH_NONE) continue;
-        const uint32_t q = p2g_slot(S.lstart[cl[r]] + rk[r]);
+        const uint32_t q = p2g_slot(cstart[r] + rk[r]);
         S.sp4[0][q] = make_float4(vv[r].w, a0[r][0], a0[r][1], a0[r][2]);
         S.sp4[1][q] = make_float4(fo[r][0], fo[r][1], fo[r][2], cc[r][0]);
         S.sp4[2][q] = make_float4(cc[r][1], cc[r][2], cc[r][3], cc[r][4]);
@@ -1513,8 +1520,6 @@ __global__ __launch_bounds__(192, P2G_CHUNK <= 384 ? 4 : 1) void k_p2g(StepParam
             if (threadIdx.x < 64) cell_cnt[(size_t)hid * NODES_PER_BLOCK + threadIdx.x] = 0u;
             continue;
         }
-        __syncthreads();  // the previous block's tile has been written out
-        for (uint32_t j = threadIdx.x; j < P2G_TILE_LDS; j += 192) S.tile[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const uint32_t block_key = keys[hid];
         float acc[9][4];
 #pragma unroll
@@ -1544,32 +1549,43 @@ __global__ __launch_bounds__(192, P2G_CHUNK <= 384 ? 4 : 1) void k_p2g(StepParam
                                          block_key, acc, tot);
             }
         }
-        __syncthreads();
+        // Each wave adds its registers into its own slab tile (sharing the staging's
+        // LDS), in 9 rounds in which its 64 lanes touch 64 distinct nodes; then each
+        // node is the sum of the slabs that cover it, in wave order.
+        __syncthreads();  // the staging's readers are done
+        float4 *slab = S.slab[kz];
+        for (uint32_t j = lane; j < P2G_SLAB_LDS; j += 64) slab[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 #pragma unroll
-        for (int ww = 0; ww < 3; ww++) {
-            if ((int)kz == ww) {
+        for (int ky = 0; ky < 3; ky++)
 #pragma unroll
-                for (int ky = 0; ky < 3; ky++)
-#pragma unroll
-                    for (int kx = 0; kx < 3; kx++) {
-                        const int q = kx + 3 * ky;
-                        const uint32_t node = p2g_tile_lds(lx + kx, ly + ky, lz + kz);
-                        float4 t = S.tile[node];
-                        t.x += acc[q][0];
-                        t.y += acc[q][1];
-                        t.z += acc[q][2];
-                        t.w += acc[q][3];
-                        S.tile[node] = t;
-                        // The next round's nodes overlap other lanes' nodes of this
-                        // round: complete this round's LDS writes first.
-                        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-                    }
+            for (int kx = 0; kx < 3; kx++) {
+                const int q = kx + 3 * ky;
+                const uint32_t node = p2g_tile_lds(lx + kx, ly + ky, lz);
+                float4 t = slab[node];
+                t.x += acc[q][0];
+                t.y += acc[q][1];
+                t.z += acc[q][2];
+                t.w += acc[q][3];
+                slab[node] = t;
+                // The next round's nodes overlap other lanes' nodes of this
+                // round: complete this round's LDS writes first.
+                __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
             }
-            __syncthreads();
-        }
+        __syncthreads();
         for (uint32_t j = threadIdx.x; j < TILE_NODES; j += 192) {
             const uint32_t x = j % 6u, y = (j / 6u) % 6u, z = j / 36u;
-            tiles[(size_t)hid * TILE_NODES + tile_index(x, y, z)] = S.tile[p2g_tile_lds(x, y, z)];
+            float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (uint32_t w = 0; w < 3; w++) {
+                if (z < w || z > w + 3u) continue;
+                const float4 u = S.slab[w][p2g_tile_lds(x, y, z - w)];
+                t.x += u.x;
+                t.y += u.y;
+                t.z += u.z;
+                t.w += u.w;
+            }
+            tiles[(size_t)hid * TILE_NODES + tile_index(x, y, z)] = t;
         }
         if (kz == 0) cell_cnt[(size_t)hid * NODES_PER_BLOCK + lane] = tot;
     }
