@@ -1480,7 +1480,7 @@ __device__ inline void p2g_chunk(const StepParams &P, P2GShared<P2G_CHUNK> &S, u
 }
 
 // Chunks of up to 384 members fit in 128 VGPRs (4 waves/SIMD; their LDS allows 5
-// workgroups per CU); 512 needs ~156 (3 waves/SIMD, as its 40 KB of LDS allow).
+// workgroups per CU); 512 needs ~156 (3 waves/SIMD, as its 35 KB of LDS allow).
 template <int P2G_CHUNK, bool FAST>
 __global__ __launch_bounds__(192, P2G_CHUNK <= 384 ? 4 : 1) void k_p2g(StepParams P, NextTable nt, const float4 *__restrict__ pos,
                                              const float4 *__restrict__ vm, const float *__restrict__ C,
