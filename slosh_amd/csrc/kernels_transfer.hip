@@ -1309,8 +1309,9 @@ __device__ void p2g_block_scan(const Members &M, uint32_t *__restrict__ blk_star
     }
 }
 
-// LDS element of node (x, y, z) of a P2G slab tile (z = 0..3), 16-byte elements. Within a z plane,
-// x < 4 maps to x + 4 y (0..23) and x >= 4 to x + 4 y + 24 (28..49), one to one;
+// LDS element of node (x, y, z) of a P2G slab tile (z = 0..3), 16-byte elements.
+// Within a z plane, x < 4 maps to x + 4 y (0..23) and x >= 4 to x + 4 y + 24
+// (28..49), one to one;
 // over any 4x4 patch of a plane (the 16 lanes of one merge access, at offsets
 // 0..2 in x and y) the element index mod 16, its LDS bank group, takes 16
 // different values (the patch's columns differ mod 4, and each column's 4 rows
