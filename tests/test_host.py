@@ -75,3 +75,21 @@ def test_served_latency_summary():
     s = SV.latency_summary(rs)
     assert s["requests"] == 3 and s["particle_substeps"] == 4000
     assert abs(s["makespan_s"] - 2.0) < 1e-12 and abs(s["p50_latency_s"] - 1.0) < 1e-12
+
+
+def test_bench_refuses_other_gpu_counts_and_empty_runs():
+    """bench.py's command line: one GPU only (a JSON line saying so, exit code 2,
+    before anything touches a GPU) and at least one timed substep."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--steps", "0"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "--steps must be at least 1" in r.stderr
